@@ -1,0 +1,153 @@
+// Schedule autotuner for the fused kernel and JSON persistence of the tuned schedules.
+//
+// The decoder's work is independent of the data (constant-time structure), so random
+// ciphertext bytes time exactly like real ones; candidates are timed on the device with
+// HIP events and the fastest schedule per parameter set is kept.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+using namespace cmd;
+
+static std::vector<FusedSchedule> candidates()
+{
+    std::vector<FusedSchedule> out;
+    for (int tile : {16, 32})
+        for (int wpg : {1, 2, 4})
+            for (int gpc : {1, 2, 4, 8}) {
+                FusedSchedule s{tile, wpg, 1, gpc};
+                // LDS per workgroup is wpg * 16.4 KB; keep resident groups within 160 KB per CU
+                if (wpg * gpc > 9) continue;
+                if (fused_supported(s)) out.push_back(s);
+            }
+    return out;
+}
+
+extern "C" int cmd_autotune(cmd_ctx *ctx, int32_t param_id, size_t count, const int32_t *key_ids, int32_t reps,
+                            cmd_schedule *best, double *best_us)
+{
+    if (!ctx || !key_ids || !count) return CMD_ERR_ARG;
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    if (reps < 1) reps = 3;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    for (size_t i = 0; i < count; i++) {
+        int32_t k = key_ids[i];
+        if (k < 0 || (size_t)k >= ctx->keys.size() || !ctx->keys[k].used || ctx->keys[k].pidx != pidx)
+            return set_error(CMD_ERR_KEY, "autotune item %zu: key %d is not a loaded key of set %d", i, (int)k,
+                             (int)param_id);
+    }
+    const ParamDesc &p = kParams[pidx];
+    const size_t ct_stride = (size_t)p.synd_bytes, e_stride = (size_t)((p.n / 8 + 3) / 4 * 4);
+    DevBuf dct, de;
+    int rc = dev_alloc(ctx, dct, count * ct_stride);
+    if (!rc) rc = dev_alloc(ctx, de, count * e_stride);
+    if (rc) {
+        dev_free(ctx, dct);
+        return rc;
+    }
+    std::vector<uint8_t> host(count * ct_stride);
+    std::mt19937_64 rng(0x5EEDu + (uint64_t)param_id);
+    for (auto &b : host) b = (uint8_t)rng();
+    hipError_t he = hipMemcpy(dct.ptr, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        dev_free(ctx, dct);
+        dev_free(ctx, de);
+        return hip_fail(he, "autotune upload");
+    }
+
+    const FusedSchedule saved = ctx->sched[pidx];
+    FusedSchedule winner = saved;
+    double winner_us = 1e30;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    for (const FusedSchedule &s : candidates()) {
+        ctx->sched[pidx] = s;
+        cmd_batch *b = nullptr;
+        rc = cmd_batch_create(ctx, CMD_MODE_FUSED, count, key_ids, ct_stride, e_stride, 0, &b);
+        if (rc) break;
+        double best_here = 1e30;
+        for (int r = 0; r < reps + 1 && !rc; r++) {
+            (void)hipEventRecord(e0, ctx->stream);
+            rc = cmd_batch_run(b, dct.ptr, de.ptr, ctx->stream);
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r > 0) best_here = std::min(best_here, (double)ms * 1000.0);  // first run warms up
+        }
+        cmd_batch_destroy(b);
+        if (rc) break;
+        if (best_here < winner_us) {
+            winner_us = best_here;
+            winner = s;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(ctx, dct);
+    dev_free(ctx, de);
+    ctx->sched[pidx] = rc ? saved : winner;
+    if (rc) return rc;
+    if (best) {
+        best->tile_cts = winner.tile_cts;
+        best->waves_per_group = winner.waves_per_group;
+        best->waves_per_ct = winner.waves_per_ct;
+        best->groups_per_cu = winner.groups_per_cu;
+    }
+    if (best_us) *best_us = winner_us;
+    return CMD_OK;
+}
+
+extern "C" int cmd_schedule_save(cmd_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    FILE *f = std::fopen(path, "w");
+    if (!f) return set_error(CMD_ERR_FILE, "cannot write %s", path);
+    std::fprintf(f, "{\n");
+    for (int i = 0; i < kNumParams; i++) {
+        const FusedSchedule &s = ctx->sched[i];
+        std::fprintf(f, "  \"%d\": {\"tile_cts\": %d, \"waves_per_group\": %d, \"waves_per_ct\": %d, "
+                        "\"groups_per_cu\": %d}%s\n",
+                     kParams[i].id, s.tile_cts, s.waves_per_group, s.waves_per_ct, s.groups_per_cu,
+                     i + 1 < kNumParams ? "," : "");
+    }
+    std::fprintf(f, "}\n");
+    return std::fclose(f) == 0 ? CMD_OK : set_error(CMD_ERR_FILE, "cannot write %s", path);
+}
+
+// Reads the format written by cmd_schedule_save (one set per line).
+extern "C" int cmd_schedule_load(cmd_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return CMD_ERR_ARG;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return set_error(CMD_ERR_FILE, "cannot read %s", path);
+    char line[512];
+    int loaded = 0;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    while (std::fgets(line, sizeof line, f)) {
+        int id, tile, wpg, wpc, gpc;
+        const char *q = std::strchr(line, '"');
+        if (!q) continue;
+        if (std::sscanf(q, "\"%d\": {\"tile_cts\": %d, \"waves_per_group\": %d, \"waves_per_ct\": %d, "
+                           "\"groups_per_cu\": %d}",
+                        &id, &tile, &wpg, &wpc, &gpc) == 5) {
+            int pidx = param_index(id);
+            FusedSchedule s{tile, wpg, wpc, gpc};
+            if (pidx >= 0 && fused_supported(s) && gpc >= 1 && gpc <= 16) {
+                ctx->sched[pidx] = s;
+                loaded++;
+            }
+        }
+    }
+    std::fclose(f);
+    return loaded ? CMD_OK : set_error(CMD_ERR_FILE, "no schedules in %s", path);
+}

@@ -1,0 +1,203 @@
+// Device-side GF(2^m) arithmetic for gfx950 (wave64, 32-bit VALU lanes).
+//
+// Two representations:
+//  * scalar: one field element per lane in the low m bits of a 32-bit register
+//    (the reference's uint16 gf, CPU/param/param6960119/include/common.h:34);
+//  * bit-sliced: m registers, register k holding bit k of 32 independent elements
+//    (one per bit position).  A product of 32 element pairs then costs m*m AND-XOR
+//    steps plus the fold of the m-1 high planes, all plain bitwise VALU work with no
+//    data-dependent addressing.
+// Field moduli: x^13+x^4+x^3+x+1 (CPU/src/common/gf_13.c:103-108) and x^12+x^3+1
+// (CPU/param/param348864/src/gf.c:116-122).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cmd {
+
+// Low part of the modulus (x^m term removed).
+template <int M>
+struct Modulus;
+template <>
+struct Modulus<13> { static constexpr uint32_t low = 0x1Bu; };
+template <>
+struct Modulus<12> { static constexpr uint32_t low = 0x9u; };
+
+// all-ones if bit k of x is set, else 0
+__device__ __forceinline__ uint32_t bitmask(uint32_t x, int k)
+{
+    return (uint32_t)__builtin_amdgcn_sbfe((int)x, k, 1);
+}
+
+// Reduces a product of degree <= 2M-2 modulo the field polynomial.
+template <int M>
+__device__ __forceinline__ uint32_t gf_reduce(uint32_t v)
+{
+    constexpr uint32_t low = Modulus<M>::low;
+    constexpr uint32_t mask = (1u << M) - 1u;
+    uint32_t hi = v >> M;
+    v &= mask;
+#pragma unroll
+    for (int b = 0; b < 5; b++)
+        if ((low >> b) & 1u) v ^= hi << b;
+    hi = v >> M;
+    v &= mask;
+#pragma unroll
+    for (int b = 0; b < 5; b++)
+        if ((low >> b) & 1u) v ^= hi << b;
+    return v;
+}
+
+// Scalar product (shift-and-add over the bits of b).
+template <int M>
+__device__ __forceinline__ uint32_t gf_mul(uint32_t a, uint32_t b)
+{
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < M; i++) acc ^= (a << i) & bitmask(b, i);
+    return gf_reduce<M>(acc);
+}
+
+// x * a for a reduced element
+template <int M>
+__device__ __forceinline__ uint32_t gf_xtime(uint32_t a)
+{
+    uint32_t v = a << 1;
+    return (v & ((1u << M) - 1u)) ^ (Modulus<M>::low & (0u - (v >> M)));
+}
+
+// Multiplication table of a fixed element a: col[k] = a * x^k.  Then a*b is the XOR of
+// col[k] over the set bits k of b (no reduction step needed).
+template <int M>
+__device__ __forceinline__ void gf_mul_cols(uint32_t a, uint32_t col[M])
+{
+    col[0] = a;
+#pragma unroll
+    for (int k = 1; k < M; k++) col[k] = gf_xtime<M>(col[k - 1]);
+}
+
+template <int M>
+__device__ __forceinline__ uint32_t gf_mul_by_cols(const uint32_t col[M], uint32_t b)
+{
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < M; k++) acc ^= col[k] & bitmask(b, k);
+    return acc;
+}
+
+// a^(2^M - 2): the inverse, with inv(0) == 0.
+template <int M>
+__device__ __forceinline__ uint32_t gf_inv(uint32_t a)
+{
+    // a^(2^(M-1)-1) by an addition chain on runs of ones, then one squaring.
+    uint32_t x1 = a;                                   // 1
+    uint32_t x2 = gf_mul<M>(gf_mul<M>(x1, x1), x1);     // 11
+    uint32_t x4 = x2;
+    x4 = gf_mul<M>(x4, x4); x4 = gf_mul<M>(x4, x4);
+    x4 = gf_mul<M>(x4, x2);                            // 1111
+    uint32_t x8 = x4;
+#pragma unroll
+    for (int i = 0; i < 4; i++) x8 = gf_mul<M>(x8, x8);
+    x8 = gf_mul<M>(x8, x4);                            // 1^8
+    uint32_t r = x8;
+    constexpr int rest = M - 1 - 8;                    // 4 (M=13) or 3 (M=12)
+    if constexpr (rest == 4) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) r = gf_mul<M>(r, r);
+        r = gf_mul<M>(r, x4);                          // 1^12
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) r = gf_mul<M>(r, r);
+        r = gf_mul<M>(r, gf_mul<M>(gf_mul<M>(x2, x2), x1));  // 1^11: append 111 = a^7
+    }
+    return gf_mul<M>(r, r);                            // 1^(M-1) 0
+}
+
+// ---- bit-sliced ------------------------------------------------------------------------
+
+// out = a * b for 32 element pairs (bit lanes).  out may alias a or b.
+template <int M>
+__device__ __forceinline__ void bs_mul(const uint32_t a[M], const uint32_t b[M], uint32_t out[M])
+{
+    constexpr uint32_t low = Modulus<M>::low;
+    uint32_t p[2 * M - 1];
+#pragma unroll
+    for (int s = 0; s < 2 * M - 1; s++) p[s] = 0;
+#pragma unroll
+    for (int i = 0; i < M; i++)
+#pragma unroll
+        for (int j = 0; j < M; j++) p[i + j] ^= a[i] & b[j];
+#pragma unroll
+    for (int s = 2 * M - 2; s >= M; s--)
+#pragma unroll
+        for (int b2 = 0; b2 < 5; b2++)
+            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+#pragma unroll
+    for (int k = 0; k < M; k++) out[k] = p[k];
+}
+
+// out = a*b + c*d for 32 element pairs each, with a single fold.
+template <int M>
+__device__ __forceinline__ void bs_mul_add2(const uint32_t a[M], const uint32_t b[M], const uint32_t c[M],
+                                            const uint32_t d[M], uint32_t out[M])
+{
+    constexpr uint32_t low = Modulus<M>::low;
+    uint32_t p[2 * M - 1];
+#pragma unroll
+    for (int s = 0; s < 2 * M - 1; s++) p[s] = 0;
+#pragma unroll
+    for (int i = 0; i < M; i++)
+#pragma unroll
+        for (int j = 0; j < M; j++) p[i + j] ^= (a[i] & b[j]) ^ (c[i] & d[j]);
+#pragma unroll
+    for (int s = 2 * M - 2; s >= M; s--)
+#pragma unroll
+        for (int b2 = 0; b2 < 5; b2++)
+            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+#pragma unroll
+    for (int k = 0; k < M; k++) out[k] = p[k];
+}
+
+// Unreduced accumulation p[0..2M-2] += a*b (carry-less, no fold).
+template <int M>
+__device__ __forceinline__ void bs_mac_wide(const uint32_t a[M], const uint32_t b[M], uint32_t p[2 * M - 1])
+{
+#pragma unroll
+    for (int i = 0; i < M; i++)
+#pragma unroll
+        for (int j = 0; j < M; j++) p[i + j] ^= a[i] & b[j];
+}
+
+// Folds an unreduced product back to m planes.
+template <int M>
+__device__ __forceinline__ void bs_fold(uint32_t p[2 * M - 1], uint32_t out[M])
+{
+    constexpr uint32_t low = Modulus<M>::low;
+#pragma unroll
+    for (int s = 2 * M - 2; s >= M; s--)
+#pragma unroll
+        for (int b2 = 0; b2 < 5; b2++)
+            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+#pragma unroll
+    for (int k = 0; k < M; k++) out[k] = p[k];
+}
+
+// a <- a * a (squaring is linear: spread then fold).
+template <int M>
+__device__ __forceinline__ void bs_sq(uint32_t a[M])
+{
+    constexpr uint32_t low = Modulus<M>::low;
+    uint32_t p[2 * M - 1];
+#pragma unroll
+    for (int s = 0; s < 2 * M - 1; s++) p[s] = (s & 1) ? 0u : a[s / 2];
+#pragma unroll
+    for (int s = 2 * M - 2; s >= M; s--)
+#pragma unroll
+        for (int b2 = 0; b2 < 5; b2++)
+            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+#pragma unroll
+    for (int k = 0; k < M; k++) a[k] = p[k];
+}
+
+}  // namespace cmd

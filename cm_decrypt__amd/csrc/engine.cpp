@@ -1,0 +1,781 @@
+// Host engine behind the C ABI: contexts, resident keys, batch planning and launch.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+namespace cmd {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return set_error(CMD_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int dev_alloc(cmd_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    dev_free(ctx, b);
+    if (!bytes) return CMD_OK;
+    if (ctx->used + bytes > ctx->budget)
+        return set_error(CMD_ERR_BUDGET, "device budget: %llu used + %zu requested > %llu",
+                         (unsigned long long)ctx->used, bytes, (unsigned long long)ctx->budget);
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    b.bytes = bytes;
+    ctx->used += bytes;
+    return CMD_OK;
+}
+
+int dev_reserve(cmd_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.ptr) return CMD_OK;
+    size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    int rc = dev_alloc(ctx, b, want);
+    if (rc == CMD_ERR_BUDGET && want > bytes) rc = dev_alloc(ctx, b, bytes);
+    return rc;
+}
+
+void dev_free(cmd_ctx *ctx, DevBuf &b)
+{
+    if (b.ptr) {
+        (void)hipFree(b.ptr);
+        ctx->used -= b.bytes;
+    }
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+FusedSchedule default_schedule(int pidx)
+{
+    (void)pidx;
+    FusedSchedule s;
+    s.tile_cts = 32;
+    s.waves_per_group = 4;
+    s.waves_per_ct = 1;
+    s.groups_per_cu = 2;
+    return s;
+}
+
+static size_t key_device_bytes(int pidx)
+{
+    const ParamDesc &p = kParams[pidx];
+    size_t table = (size_t)p.sb * 2 * p.t * 2;
+    size_t L = (size_t)p.n * 2;
+    size_t lb = (size_t)p.m * lbits_stride(pidx) * 4;
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    return up(table) + up(L) + up(lb);
+}
+
+static int upload_key_slot(cmd_ctx *ctx, int32_t id)
+{
+    CMD_HIP(hipMemcpy(ctx->d_keys + id, &ctx->keys[id].dev, sizeof(KeyDev), hipMemcpyHostToDevice));
+    return CMD_OK;
+}
+
+// ---- batch planning ---------------------------------------------------------------------
+
+void batch_release(cmd_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    dev_free(b->ctx, b->d_plan);
+    dev_free(b->ctx, b->d_s);
+    dev_free(b->ctx, b->d_loc);
+}
+
+int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, const int32_t *key_ids,
+               std::vector<FusedTile> &tiles, std::vector<int32_t> &items, std::vector<int32_t> &item_key,
+               int *launch_tile)
+{
+    std::vector<int32_t> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+        int pa = ctx->keys[key_ids[a]].pidx, pc = ctx->keys[key_ids[c]].pidx;
+        if (pa != pc) return pa > pc;  // heavier sets (larger index here) first
+        return key_ids[a] < key_ids[c];
+    });
+    tiles.clear();
+    items.clear();
+    item_key.clear();
+    int widest = 0;
+    // tiles of up to tile_cts consecutive items of one set (a tile may span keys of that set)
+    for (size_t i = 0; i < count;) {
+        const int pidx = ctx->keys[key_ids[order[i]]].pidx;
+        const int width = sched[pidx].tile_cts;
+        size_t j = i;
+        while (j < count && (int)(j - i) < width && ctx->keys[key_ids[order[j]]].pidx == pidx) j++;
+        FusedTile t;
+        t.pidx = pidx;
+        t.first = (int32_t)items.size();
+        t.count = (int32_t)(j - i);
+        t.pad = 0;
+        tiles.push_back(t);
+        widest = std::max(widest, width);
+        for (size_t q = i; q < j; q++) {
+            items.push_back(order[q]);
+            item_key.push_back(key_ids[order[q]]);
+        }
+        i = j;
+    }
+    if (launch_tile) *launch_tile = widest ? widest : 32;
+    return CMD_OK;
+}
+
+FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<FusedTile> &tiles, int launch_tile)
+{
+    FusedSchedule s = tiles.empty() ? default_schedule(0) : sched[tiles[0].pidx];
+    s.tile_cts = launch_tile;
+    return s;
+}
+
+int batch_plan(cmd_batch *b, const int32_t *key_ids)
+{
+    cmd_ctx *ctx = b->ctx;
+    const size_t count = b->count;
+    // validate keys, find max sizes
+    size_t max_ct = 0, max_e = 0;
+    for (size_t i = 0; i < count; i++) {
+        int32_t k = key_ids[i];
+        if (k < 0 || (size_t)k >= ctx->keys.size() || !ctx->keys[k].used)
+            return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)k);
+        const ParamDesc &p = kParams[ctx->keys[k].pidx];
+        max_ct = std::max(max_ct, (size_t)p.synd_bytes);
+        max_e = std::max(max_e, (size_t)p.n / 8);
+    }
+    if (b->ct_stride < max_ct || b->e_stride < max_e)
+        return set_error(CMD_ERR_ARG, "strides (%zu, %zu) below item sizes (%zu, %zu)", b->ct_stride, b->e_stride,
+                         max_ct, max_e);
+
+    // order items by (set, key), keeping input order inside a key
+    std::vector<int32_t> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+        int pa = ctx->keys[key_ids[a]].pidx, pc = ctx->keys[key_ids[c]].pidx;
+        if (pa != pc) return pa < pc;
+        return key_ids[a] < key_ids[c];
+    });
+
+    b->groups.clear();
+    b->tiles.clear();
+    b->items.clear();
+    b->item_key.clear();
+    b->max_group = 0;
+    b->max_t = 0;
+    for (size_t i = 0; i < count;) {
+        size_t j = i;
+        while (j < count && key_ids[order[j]] == key_ids[order[i]]) j++;
+        int key = key_ids[order[i]];
+        int pidx = ctx->keys[key].pidx;
+        b->groups.push_back({pidx, key, (int)i, (int)(j - i)});
+        b->max_group = std::max(b->max_group, (int)(j - i));
+        b->max_t = std::max(b->max_t, (int)kParams[pidx].t);
+        i = j;
+    }
+
+    if (b->mode == CMD_MODE_FUSED) {
+        int rc = plan_fused(ctx, b->sched, count, key_ids, b->tiles, b->items, b->item_key, &b->launch_tile);
+        if (rc) return rc;
+    } else {
+        b->items.assign(order.begin(), order.end());
+    }
+
+    // upload: [items | item_key | tiles | counter]
+    size_t n_items = b->items.size();
+    size_t off_items = 0;
+    size_t off_keys = off_items + ((n_items * 4 + 255) / 256) * 256;
+    size_t off_tiles = off_keys + ((b->item_key.size() * 4 + 255) / 256) * 256;
+    size_t off_ctr = off_tiles + ((b->tiles.size() * sizeof(FusedTile) + 255) / 256) * 256;
+    size_t total = off_ctr + 256;
+    int rc = dev_alloc(ctx, b->d_plan, total);
+    if (rc) return rc;
+    char *base = (char *)b->d_plan.ptr;
+    b->d_idx = (int32_t *)(base + off_items);
+    b->d_items = (int32_t *)(base + off_items);
+    b->d_item_key = (int32_t *)(base + off_keys);
+    b->d_tiles = (FusedTile *)(base + off_tiles);
+    b->d_counter = (uint32_t *)(base + off_ctr);
+    if (n_items) CMD_HIP(hipMemcpy(b->d_items, b->items.data(), n_items * 4, hipMemcpyHostToDevice));
+    if (!b->item_key.empty())
+        CMD_HIP(hipMemcpy(b->d_item_key, b->item_key.data(), b->item_key.size() * 4, hipMemcpyHostToDevice));
+    if (!b->tiles.empty())
+        CMD_HIP(hipMemcpy(b->d_tiles, b->tiles.data(), b->tiles.size() * sizeof(FusedTile), hipMemcpyHostToDevice));
+
+    if (b->mode == CMD_MODE_PLAIN && b->max_group > 0) {
+        rc = dev_alloc(ctx, b->d_s, (size_t)b->max_group * 2 * b->max_t * 2);
+        if (!rc) rc = dev_alloc(ctx, b->d_loc, (size_t)b->max_group * (b->max_t + 1) * 2);
+        if (rc) return rc;
+    }
+    return CMD_OK;
+}
+
+int batch_enqueue(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
+{
+    cmd_ctx *ctx = b->ctx;
+    const uint8_t *ct = (const uint8_t *)d_ct;
+    uint8_t *e = (uint8_t *)d_e;
+    if (b->mode == CMD_MODE_PLAIN) {
+        for (const auto &g : b->groups) {
+            const KeyDev &key = ctx->keys[g.key].dev;
+            const int32_t *idx = b->d_idx + g.first;
+            CMD_HIP(launch_plain_synd(g.pidx, key, g.count, idx, ct, b->ct_stride, (uint16_t *)b->d_s.ptr, st));
+            CMD_HIP(launch_plain_bm(g.pidx, g.count, (const uint16_t *)b->d_s.ptr, (uint16_t *)b->d_loc.ptr, st));
+            CMD_HIP(launch_plain_chien(g.pidx, key, g.count, idx, (const uint16_t *)b->d_loc.ptr, e, b->e_stride,
+                                       st));
+        }
+        return CMD_OK;
+    }
+    if (b->tiles.empty()) return CMD_OK;
+    FusedArgs a;
+    a.tiles = b->d_tiles;
+    a.n_tiles = (int32_t)b->tiles.size();
+    a.pad0 = 0;
+    a.items = b->d_items;
+    a.item_key = b->d_item_key;
+    a.keys = ctx->d_keys;
+    a.ct = ct;
+    a.e = e;
+    a.ct_stride = b->ct_stride;
+    a.e_stride = b->e_stride;
+    a.tile_counter = b->d_counter;
+    const FusedSchedule s = launch_schedule(b->sched, b->tiles, b->launch_tile);
+    CMD_HIP(launch_zero_u32(b->d_counter, 1, st));
+    CMD_HIP(launch_fused(a, s, ctx->n_cu, st));
+    return CMD_OK;
+}
+
+}  // namespace cmd
+
+using namespace cmd;
+
+// ---- C ABI ------------------------------------------------------------------------------
+
+extern "C" const char *cmd_strerror(int code)
+{
+    switch (code) {
+    case CMD_OK: return "ok";
+    case CMD_ERR_FILE: return "file open/read/write error";
+    case CMD_ERR_CRYPTO: return "invalid count or key bytes";
+    case CMD_ERR_PARAM: return "unknown parameter set";
+    case CMD_ERR_ARG: return "invalid argument";
+    case CMD_ERR_HIP: return "HIP runtime error";
+    case CMD_ERR_NODEV: return "no GPU available";
+    case CMD_ERR_BUDGET: return "device memory budget exceeded";
+    case CMD_ERR_KEY: return "unknown key";
+    case CMD_ERR_FULL: return "queue full";
+    case CMD_ERR_NOMEM: return "out of host memory";
+    case CMD_ERR_SINGULAR: return "no systematic form";
+    case CMD_ERR_PENDING: return "request pending";
+    case CMD_ERR_TIMEOUT: return "timed out";
+    }
+    return "unknown error";
+}
+
+extern "C" const char *cmd_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int cmd_param_get(int32_t param_id, cmd_param_info *out)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    if (!out) return CMD_ERR_ARG;
+    const ParamDesc &p = kParams[pidx];
+    out->id = p.id;
+    out->m = p.m;
+    out->n = p.n;
+    out->t = p.t;
+    out->ct_bytes = p.synd_bytes;
+    out->sb = p.sb;
+    out->sk_bytes = p.sk_bytes;
+    out->cond_bytes = p.cond_bytes;
+    out->e_bytes = p.n / 8;
+    return CMD_OK;
+}
+
+extern "C" int cmd_support_gen(int32_t param_id, const uint8_t *cond, uint16_t *L)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    if (!cond || !L) return CMD_ERR_ARG;
+    return support_gen(pidx, cond, L);
+}
+
+extern "C" int cmd_key_expand_host(int32_t param_id, const uint8_t *sk, size_t sk_len, uint16_t *g, uint16_t *L,
+                                   uint16_t *table)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    ExpandedKey K;
+    int rc = expand_key(pidx, sk, sk_len, K);
+    if (rc) return set_error(rc, "secret key of %zu bytes, %d expected", sk_len, kParams[pidx].sk_bytes);
+    if (g) std::memcpy(g, K.g.data(), K.g.size() * 2);
+    if (L) std::memcpy(L, K.L.data(), K.L.size() * 2);
+    if (table) std::memcpy(table, K.table.data(), K.table.size() * 2);
+    return CMD_OK;
+}
+
+extern "C" int cmd_errorstream_format(int32_t param_id, const uint8_t *e, size_t e_stride, size_t count, char *buf,
+                                      size_t cap, size_t *written)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    const int n = kParams[pidx].n;
+    if ((!e && count) || e_stride < (size_t)n / 8) return CMD_ERR_ARG;
+    size_t pos = 0;
+    char tmp[8];
+    for (size_t k = 0; k < count; k++) {
+        const uint8_t *v = e + k * e_stride;
+        for (int byte = 0; byte < n / 8; byte++) {
+            uint32_t bits = v[byte];
+            while (bits) {
+                int i = byte * 8 + __builtin_ctz(bits);
+                bits &= bits - 1;
+                int len = 0;
+                do { tmp[len++] = (char)('0' + i % 10); i /= 10; } while (i);
+                if (pos < cap) buf[pos] = ' ';
+                pos++;
+                while (len) {
+                    char c = tmp[--len];
+                    if (pos < cap) buf[pos] = c;
+                    pos++;
+                }
+            }
+        }
+        if (pos < cap) buf[pos] = '\n';
+        pos++;
+    }
+    if (written) *written = pos;
+    return pos <= cap || cap == 0 ? CMD_OK : set_error(CMD_ERR_ARG, "buffer of %zu bytes, %zu needed", cap, pos);
+}
+
+extern "C" int cmd_errorstream_write(int32_t param_id, const char *path, const uint8_t *e, size_t e_stride,
+                                     size_t count, int append)
+{
+    size_t need = 0;
+    int rc = cmd_errorstream_format(param_id, e, e_stride, count, nullptr, 0, &need);
+    if (rc) return rc;
+    std::vector<char> buf(need);
+    rc = cmd_errorstream_format(param_id, e, e_stride, count, buf.data(), buf.size(), &need);
+    if (rc) return rc;
+    FILE *f = std::fopen(path, append ? "ab" : "wb");
+    if (!f) return set_error(CMD_ERR_FILE, "cannot open %s", path);
+    size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+    int cl = std::fclose(f);
+    if (w != buf.size() || cl != 0) return set_error(CMD_ERR_FILE, "short write to %s", path);
+    return CMD_OK;
+}
+
+extern "C" int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx **out)
+{
+    if (!out) return CMD_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(CMD_ERR_NODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return set_error(CMD_ERR_NODEV, "device %d of %d", (int)device, ndev);
+    CMD_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CMD_HIP(hipGetDeviceProperties(&prop, device));
+    std::unique_ptr<cmd_ctx> ctx(new cmd_ctx());
+    ctx->device = device;
+    ctx->n_cu = prop.multiProcessorCount;
+    size_t free_b = 0, total_b = 0;
+    CMD_HIP(hipMemGetInfo(&free_b, &total_b));
+    ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
+    CMD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    CMD_HIP(hipEventCreate(&ctx->ev0));
+    CMD_HIP(hipEventCreate(&ctx->ev1));
+    for (int i = 0; i < kNumParams; i++) ctx->sched[i] = default_schedule(i);
+    CMD_HIP(hipMalloc(&ctx->d_keys, kMaxKeys * sizeof(KeyDev)));
+    *out = ctx.release();
+    return CMD_OK;
+}
+
+extern "C" int cmd_ctx_destroy(cmd_ctx *ctx)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &k : ctx->keys)
+        if (k.mem) (void)hipFree(k.mem);
+    if (ctx->d_keys) (void)hipFree(ctx->d_keys);
+    for (DevBuf *b : {&ctx->ct, &ctx->e, &ctx->s, &ctx->loc, &ctx->idx, &ctx->plan}) dev_free(ctx, *b);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CMD_OK;
+}
+
+extern "C" int cmd_ctx_memory(cmd_ctx *ctx, uint64_t *used, uint64_t *budget)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    if (used) *used = ctx->used;
+    if (budget) *budget = ctx->budget;
+    return CMD_OK;
+}
+
+extern "C" int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    if (bytes) *bytes = key_device_bytes(pidx);
+    return CMD_OK;
+}
+
+extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, size_t sk_len, int32_t *key_id)
+{
+    if (!ctx || !sk || !key_id) return CMD_ERR_ARG;
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    ExpandedKey K;
+    int rc = expand_key(pidx, sk, sk_len, K);
+    if (rc) return set_error(rc, "secret key of %zu bytes, %d expected", sk_len, kParams[pidx].sk_bytes);
+
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    const size_t bytes = key_device_bytes(pidx);
+    if (ctx->used + bytes > ctx->budget)
+        return set_error(CMD_ERR_BUDGET, "key of %zu bytes exceeds the budget (%llu of %llu used)", bytes,
+                         (unsigned long long)ctx->used, (unsigned long long)ctx->budget);
+    void *mem = nullptr;
+    CMD_HIP(hipMalloc(&mem, bytes));
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    const ParamDesc &p = kParams[pidx];
+    char *base = (char *)mem;
+    size_t off_L = up((size_t)p.sb * 2 * p.t * 2), off_lb = off_L + up((size_t)p.n * 2);
+    hipError_t e1 = hipMemcpy(base, K.table.data(), K.table.size() * 2, hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(base + off_L, K.L.data(), K.L.size() * 2, hipMemcpyHostToDevice);
+    hipError_t e3 = hipMemcpy(base + off_lb, K.lbits.data(), K.lbits.size() * 4, hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        (void)hipFree(mem);
+        return hip_fail(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "key upload");
+    }
+    KeySlot slot;
+    slot.used = true;
+    slot.pidx = pidx;
+    slot.mem = mem;
+    slot.bytes = bytes;
+    slot.dev.table = (const uint16_t *)base;
+    slot.dev.L = (const uint16_t *)(base + off_L);
+    slot.dev.lbits = (const uint32_t *)(base + off_lb);
+    slot.dev.pidx = pidx;
+    slot.dev.lstride = lbits_stride(pidx);
+    int32_t id = -1;
+    for (size_t i = 0; i < ctx->keys.size(); i++)
+        if (!ctx->keys[i].used) { id = (int32_t)i; break; }
+    if (id < 0) {
+        if (ctx->keys.size() >= kMaxKeys) {
+            (void)hipFree(mem);
+            return set_error(CMD_ERR_BUDGET, "all %zu key slots are in use", kMaxKeys);
+        }
+        id = (int32_t)ctx->keys.size();
+        ctx->keys.push_back(slot);
+    } else {
+        ctx->keys[id] = slot;
+    }
+    ctx->used += bytes;
+    rc = upload_key_slot(ctx, id);
+    if (rc) return rc;
+    *key_id = id;
+    return CMD_OK;
+}
+
+extern "C" int cmd_key_unload(cmd_ctx *ctx, int32_t key_id)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    if (key_id < 0 || (size_t)key_id >= ctx->keys.size() || !ctx->keys[key_id].used)
+        return set_error(CMD_ERR_KEY, "key id %d is not loaded", (int)key_id);
+    if (ctx->keys[key_id].inflight > 0)
+        return set_error(CMD_ERR_KEY, "key id %d has %d requests in flight", (int)key_id, ctx->keys[key_id].inflight);
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    KeySlot &k = ctx->keys[key_id];
+    (void)hipFree(k.mem);
+    ctx->used -= k.bytes;
+    k = KeySlot();
+    return upload_key_slot(ctx, key_id);
+}
+
+static int make_batch(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids, size_t ct_stride,
+                      size_t e_stride, int32_t use_graph, cmd_batch **out)
+{
+    if (mode != CMD_MODE_PLAIN && mode != CMD_MODE_FUSED) return set_error(CMD_ERR_ARG, "mode %d", (int)mode);
+    if (count && !key_ids) return CMD_ERR_ARG;
+    if (count > (size_t)INT32_MAX) return set_error(CMD_ERR_ARG, "count %zu too large", count);
+    std::unique_ptr<cmd_batch> b(new cmd_batch());
+    b->ctx = ctx;
+    b->mode = mode;
+    b->count = count;
+    b->ct_stride = ct_stride;
+    b->e_stride = e_stride;
+    b->use_graph = use_graph != 0;
+    for (int i = 0; i < kNumParams; i++) b->sched[i] = ctx->sched[i];
+    int rc = batch_plan(b.get(), key_ids);
+    if (rc) {
+        batch_release(b.get());
+        return rc;
+    }
+    *out = b.release();
+    return CMD_OK;
+}
+
+static int run_batch(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
+{
+    cmd_ctx *ctx = b->ctx;
+    if (ctx->timing) CMD_HIP(hipEventRecord(ctx->ev0, st));
+    if (b->use_graph) {
+        if (!b->exec || b->graph_ct != d_ct || b->graph_e != d_e || b->graph_stream != st) {
+            if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+            CMD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = batch_enqueue(b, d_ct, d_e, st);
+            hipGraph_t graph = nullptr;
+            hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
+            b->graph = graph;
+            CMD_HIP(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+            b->graph_ct = d_ct;
+            b->graph_e = d_e;
+            b->graph_stream = st;
+        }
+        CMD_HIP(hipGraphLaunch(b->exec, st));
+    } else {
+        int rc = batch_enqueue(b, d_ct, d_e, st);
+        if (rc) return rc;
+    }
+    if (ctx->timing) {
+        CMD_HIP(hipEventRecord(ctx->ev1, st));
+        CMD_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        CMD_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->timing_us += ms * 1000.0;
+        ctx->timing_n++;
+    }
+    return CMD_OK;
+}
+
+extern "C" int cmd_batch_create(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids, size_t ct_stride,
+                                size_t e_stride, int32_t use_graph, cmd_batch **out)
+{
+    if (!ctx || !out) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    return make_batch(ctx, mode, count, key_ids, ct_stride, e_stride, use_graph, out);
+}
+
+extern "C" int cmd_batch_run(cmd_batch *b, const void *d_ct, void *d_e, void *stream)
+{
+    if (!b || (b->count && (!d_ct || !d_e))) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+    CMD_HIP(hipSetDevice(b->ctx->device));
+    return run_batch(b, d_ct, d_e, stream ? (hipStream_t)stream : b->ctx->stream);
+}
+
+extern "C" int cmd_batch_destroy(cmd_batch *b)
+{
+    if (!b) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    batch_release(b);
+    delete b;
+    return CMD_OK;
+}
+
+extern "C" int cmd_decrypt_device(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids, const void *d_ct,
+                                  size_t ct_stride, void *d_e, size_t e_stride, void *stream)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    if (!count) return CMD_OK;
+    if (!d_ct || !d_e) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    cmd_batch *b = nullptr;
+    int rc = make_batch(ctx, mode, count, key_ids, ct_stride, e_stride, 0, &b);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    rc = run_batch(b, d_ct, d_e, st);
+    // the plan buffers must outlive the launches
+    hipError_t se = hipStreamSynchronize(st);
+    batch_release(b);
+    delete b;
+    if (rc) return rc;
+    if (se != hipSuccess) return hip_fail(se, "hipStreamSynchronize");
+    return CMD_OK;
+}
+
+extern "C" int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids, const uint8_t *ct,
+                           size_t ct_stride, uint8_t *e_out, size_t e_stride)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    if (!count) return CMD_OK;
+    if (!ct || !e_out || !key_ids) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    int rc = dev_reserve(ctx, ctx->ct, count * ct_stride);
+    if (!rc) rc = dev_reserve(ctx, ctx->e, count * e_stride);
+    if (rc) return rc;
+    CMD_HIP(hipMemcpyAsync(ctx->ct.ptr, ct, count * ct_stride, hipMemcpyHostToDevice, ctx->stream));
+    rc = cmd_decrypt_device(ctx, mode, count, key_ids, ctx->ct.ptr, ct_stride, ctx->e.ptr, e_stride, ctx->stream);
+    if (rc) return rc;
+    CMD_HIP(hipMemcpyAsync(e_out, ctx->e.ptr, count * e_stride, hipMemcpyDeviceToHost, ctx->stream));
+    CMD_HIP(hipStreamSynchronize(ctx->stream));
+    return CMD_OK;
+}
+
+// ---- per-stage entry points ----------------------------------------------------------------
+
+static int check_key(cmd_ctx *ctx, int32_t key_id)
+{
+    if (key_id < 0 || (size_t)key_id >= ctx->keys.size() || !ctx->keys[key_id].used)
+        return set_error(CMD_ERR_KEY, "key id %d is not loaded", (int)key_id);
+    return CMD_OK;
+}
+
+extern "C" int cmd_stage_synd(cmd_ctx *ctx, int32_t key_id, size_t count, const uint8_t *ct, size_t ct_stride,
+                              uint16_t *s_out)
+{
+    if (!ctx || (count && (!ct || !s_out))) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    int rc = check_key(ctx, key_id);
+    if (rc) return rc;
+    if (!count) return CMD_OK;
+    CMD_HIP(hipSetDevice(ctx->device));
+    const KeySlot &k = ctx->keys[key_id];
+    const ParamDesc &p = kParams[k.pidx];
+    if (ct_stride < (size_t)p.synd_bytes) return CMD_ERR_ARG;
+    rc = dev_reserve(ctx, ctx->ct, count * ct_stride);
+    if (!rc) rc = dev_reserve(ctx, ctx->s, count * 2 * p.t * 2);
+    if (rc) return rc;
+    CMD_HIP(hipMemcpyAsync(ctx->ct.ptr, ct, count * ct_stride, hipMemcpyHostToDevice, ctx->stream));
+    CMD_HIP(launch_plain_synd(k.pidx, k.dev, (int)count, nullptr, (const uint8_t *)ctx->ct.ptr, ct_stride,
+                              (uint16_t *)ctx->s.ptr, ctx->stream));
+    CMD_HIP(hipMemcpyAsync(s_out, ctx->s.ptr, count * 2 * p.t * 2, hipMemcpyDeviceToHost, ctx->stream));
+    CMD_HIP(hipStreamSynchronize(ctx->stream));
+    return CMD_OK;
+}
+
+extern "C" int cmd_stage_bm(cmd_ctx *ctx, int32_t param_id, size_t count, const uint16_t *s, uint16_t *loc_out)
+{
+    if (!ctx || (count && (!s || !loc_out))) return CMD_ERR_ARG;
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    if (!count) return CMD_OK;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    const ParamDesc &p = kParams[pidx];
+    int rc = dev_reserve(ctx, ctx->s, count * 2 * p.t * 2);
+    if (!rc) rc = dev_reserve(ctx, ctx->loc, count * (p.t + 1) * 2);
+    if (rc) return rc;
+    CMD_HIP(hipMemcpyAsync(ctx->s.ptr, s, count * 2 * p.t * 2, hipMemcpyHostToDevice, ctx->stream));
+    CMD_HIP(launch_plain_bm(pidx, (int)count, (const uint16_t *)ctx->s.ptr, (uint16_t *)ctx->loc.ptr, ctx->stream));
+    CMD_HIP(hipMemcpyAsync(loc_out, ctx->loc.ptr, count * (p.t + 1) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    CMD_HIP(hipStreamSynchronize(ctx->stream));
+    return CMD_OK;
+}
+
+extern "C" int cmd_stage_root(cmd_ctx *ctx, int32_t key_id, size_t count, const uint16_t *loc, uint8_t *e_out,
+                              size_t e_stride)
+{
+    if (!ctx || (count && (!loc || !e_out))) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    int rc = check_key(ctx, key_id);
+    if (rc) return rc;
+    if (!count) return CMD_OK;
+    CMD_HIP(hipSetDevice(ctx->device));
+    const KeySlot &k = ctx->keys[key_id];
+    const ParamDesc &p = kParams[k.pidx];
+    if (e_stride < (size_t)p.n / 8) return CMD_ERR_ARG;
+    rc = dev_reserve(ctx, ctx->loc, count * (p.t + 1) * 2);
+    if (!rc) rc = dev_reserve(ctx, ctx->e, count * e_stride);
+    if (rc) return rc;
+    CMD_HIP(hipMemcpyAsync(ctx->loc.ptr, loc, count * (p.t + 1) * 2, hipMemcpyHostToDevice, ctx->stream));
+    CMD_HIP(launch_plain_chien(k.pidx, k.dev, (int)count, nullptr, (const uint16_t *)ctx->loc.ptr,
+                               (uint8_t *)ctx->e.ptr, e_stride, ctx->stream));
+    CMD_HIP(hipMemcpyAsync(e_out, ctx->e.ptr, count * e_stride, hipMemcpyDeviceToHost, ctx->stream));
+    CMD_HIP(hipStreamSynchronize(ctx->stream));
+    return CMD_OK;
+}
+
+// ---- schedules and timing --------------------------------------------------------------------
+
+extern "C" int cmd_schedule_get(cmd_ctx *ctx, int32_t param_id, cmd_schedule *out)
+{
+    int pidx = param_index(param_id);
+    if (!ctx || !out) return CMD_ERR_ARG;
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    const FusedSchedule &s = ctx->sched[pidx];
+    out->tile_cts = s.tile_cts;
+    out->waves_per_group = s.waves_per_group;
+    out->waves_per_ct = s.waves_per_ct;
+    out->groups_per_cu = s.groups_per_cu;
+    return CMD_OK;
+}
+
+extern "C" int cmd_schedule_set(cmd_ctx *ctx, int32_t param_id, const cmd_schedule *in)
+{
+    int pidx = param_index(param_id);
+    if (!ctx || !in) return CMD_ERR_ARG;
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    FusedSchedule s{in->tile_cts, in->waves_per_group, in->waves_per_ct, in->groups_per_cu};
+    if (!fused_supported(s) || s.groups_per_cu < 1 || s.groups_per_cu > 16)
+        return set_error(CMD_ERR_ARG, "schedule (%d, %d, %d, %d) not available", s.tile_cts, s.waves_per_group,
+                         s.waves_per_ct, s.groups_per_cu);
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    ctx->sched[pidx] = s;
+    return CMD_OK;
+}
+
+extern "C" int cmd_set_kernel_timing(cmd_ctx *ctx, int32_t on)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    ctx->timing = on != 0;
+    ctx->timing_us = 0;
+    ctx->timing_n = 0;
+    return CMD_OK;
+}
+
+extern "C" int cmd_kernel_timing(cmd_ctx *ctx, double *avg_us, uint64_t *launches)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    if (avg_us) *avg_us = ctx->timing_n ? ctx->timing_us / (double)ctx->timing_n : 0.0;
+    if (launches) *launches = ctx->timing_n;
+    ctx->timing_us = 0;
+    ctx->timing_n = 0;
+    return CMD_OK;
+}
+
+extern "C" const char *cmd_build_info(void)
+{
+    return "cm_decrypt__amd gfx950: plain(synd/bm/chien, uint16 GF) fused(tile 16|32, waves/group 1|2|4, "
+           "bit-sliced BM + root search)";
+}

@@ -1,0 +1,128 @@
+// Host engine internals shared by the C ABI, the batch planner and the server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cm_decrypt.h"
+#include "kernels.h"
+#include "keys.h"
+#include "params.h"
+
+namespace cmd {
+
+int set_error(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+#define CMD_HIP(call)                                   \
+    do {                                                \
+        hipError_t _e = (call);                         \
+        if (_e != hipSuccess) return hip_fail(_e, #call); \
+    } while (0)
+
+// Device allocation charged against a context budget.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+constexpr size_t kMaxKeys = 16384;  // resident key slots per context
+
+struct KeySlot {
+    bool used = false;
+    int pidx = -1;
+    int inflight = 0;      // server requests referencing the key (guarded by the context lock)
+    KeyDev dev{};
+    void *mem = nullptr;   // one allocation holding table, L, lbits
+    size_t bytes = 0;
+};
+
+struct Engine;
+
+}  // namespace cmd
+
+// Opaque handles of the C ABI.
+struct cmd_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, used = 0;
+    std::recursive_mutex lock;
+
+    std::vector<cmd::KeySlot> keys;
+    cmd::KeyDev *d_keys = nullptr;  // kMaxKeys descriptors, slot i mirrors keys[i].dev
+
+    // work buffers of cmd_decrypt / cmd_decrypt_device (grow on demand)
+    cmd::DevBuf ct, e, s, loc, idx, plan;
+    uint32_t *d_counter = nullptr;
+
+    cmd::FusedSchedule sched[cmd::kNumParams];
+
+    bool timing = false;
+    double timing_us = 0.0;
+    uint64_t timing_n = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct cmd_batch {
+    cmd_ctx *ctx = nullptr;
+    int mode = 0;
+    size_t count = 0;
+    size_t ct_stride = 0, e_stride = 0;
+    bool use_graph = false;
+
+    // plain: groups of one key
+    struct Group { int pidx; int key; int first; int count; };
+    std::vector<Group> groups;
+    int max_group = 0;
+    int max_t = 0;
+
+    // fused: tile list and per-slot item/key arrays
+    std::vector<cmd::FusedTile> tiles;
+    std::vector<int32_t> items, item_key;
+    int launch_tile = 32;      // kernel tile width: the widest tile of the batch
+    cmd::DevBuf d_plan;       // device copy of idx / tiles / items / keys
+    int32_t *d_idx = nullptr; // plain: item order (grouped)
+    cmd::FusedTile *d_tiles = nullptr;
+    int32_t *d_items = nullptr, *d_item_key = nullptr;
+    cmd::DevBuf d_s, d_loc;   // plain intermediates
+    uint32_t *d_counter = nullptr;
+    cmd::FusedSchedule sched[cmd::kNumParams];
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *graph_ct = nullptr;
+    void *graph_e = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace cmd {
+
+int dev_alloc(cmd_ctx *ctx, DevBuf &b, size_t bytes);    // exact size, charged
+int dev_reserve(cmd_ctx *ctx, DevBuf &b, size_t bytes);  // grow only
+void dev_free(cmd_ctx *ctx, DevBuf &b);
+
+// Host plan of a fused launch over `count` items: tiles of one parameter set each (sorted
+// by set, then key, largest sets first) with per-slot item index and key.  Returns the
+// widest tile in *launch_tile.
+int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, const int32_t *key_ids,
+               std::vector<FusedTile> &tiles, std::vector<int32_t> &items, std::vector<int32_t> &item_key,
+               int *launch_tile);
+// Schedule a fused launch uses: tile width of the batch, waves/groups of its heaviest set.
+FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<FusedTile> &tiles, int launch_tile);
+
+// Builds the plan of a batch (host arrays) and uploads it.
+int batch_plan(cmd_batch *b, const int32_t *key_ids);
+// Enqueues the batch's launches on `st` (no graph).
+int batch_enqueue(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st);
+void batch_release(cmd_batch *b);
+
+FusedSchedule default_schedule(int pidx);
+
+}  // namespace cmd

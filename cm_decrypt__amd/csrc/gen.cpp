@@ -1,0 +1,381 @@
+// Seeded test-vector generator: key pairs and ciphertexts in the reference's formats.
+//
+// Formats follow kem/common/core/operations.c:36-107 (secret-key layout) and
+// encrypt.c:40-120 (ciphertext = systematic parity check times e).  The randomness is
+// this library's own seeded xoshiro256** stream, not the NIST AES DRBG.
+//
+//  * Goppa polynomial: minimal polynomial over GF(2^m) of a random element of
+//    GF(2^m)[y]/F(y) (the method of kem/common/core/sk_gen.c), with the extension
+//    moduli F of the five sets (kem/variant_<p>/gf.c GF_mul reductions).
+//  * Support: a seeded permutation pi of GF(2^m); L_i = bitrev(pi_i) (kem/common/core/pk_gen.c:69-70).
+//  * Condition bits: Benes routing of pi by the looping algorithm; checked by re-applying
+//    the network (as controlbitsfrompermutation does, kem/common/core/controlbits.c:186-218).
+//  * Ciphertext: with H the m*t x n binary parity-check matrix of (g, L) and A its first
+//    m*t columns, ct = A^-1 H e.  Then v = (ct, 0...0) satisfies H v = H e, which is what
+//    the systematic-form syndrome (encrypt.c:87-120) gives.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/cm_decrypt.h"
+#include "host_gf.h"
+#include "keys.h"
+#include "params.h"
+
+namespace cmd {
+
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed)
+    {
+        for (int i = 0; i < 4; i++) {  // splitmix64 expansion
+            seed += 0x9E3779B97F4A7C15ull;
+            uint64_t z = seed;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next()
+    {
+        uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+        s[2] ^= t; s[3] = rotl(s[3], 45);
+        return r;
+    }
+    uint32_t below(uint32_t bound)  // unbiased in [0, bound)
+    {
+        uint64_t lim = (~0ull) - ((~0ull) % bound);
+        uint64_t x;
+        do { x = next(); } while (x >= lim);
+        return (uint32_t)(x % bound);
+    }
+};
+
+// Extension modulus F(y) = y^t + sum taps (coefficient, power), kem/variant_<p>/gf.c.
+struct ExtTap { int power; uint16_t coef; };
+static void ext_taps(int pidx, std::vector<ExtTap> &taps)
+{
+    switch (kParams[pidx].id) {
+    case 348864: taps = {{3, 1}, {1, 1}, {0, 2}}; break;              // y^64 + y^3 + y + z
+    case 460896: taps = {{10, 1}, {9, 1}, {6, 1}, {0, 1}}; break;      // y^96 + y^10 + y^9 + y^6 + 1
+    case 6960119: taps = {{8, 1}, {0, 1}}; break;                       // y^119 + y^8 + 1
+    default: taps = {{7, 1}, {2, 1}, {1, 1}, {0, 1}}; break;           // y^128 + y^7 + y^2 + y + 1
+    }
+}
+
+// out = a * b mod F in GF(2^m)[y]/F(y), t coefficients each.
+static void ext_mul(const HostField &F, int t, const std::vector<ExtTap> &taps, const uint16_t *a,
+                    const uint16_t *b, uint16_t *out)
+{
+    std::vector<uint16_t> prod(2 * t - 1, 0);
+    for (int i = 0; i < t; i++) {
+        if (!a[i]) continue;
+        for (int j = 0; j < t; j++) prod[i + j] ^= F.mul(a[i], b[j]);
+    }
+    for (int i = 2 * t - 2; i >= t; i--) {
+        uint16_t c = prod[i];
+        if (!c) continue;
+        for (const ExtTap &tp : taps) prod[i - t + tp.power] ^= F.mul(c, tp.coef);
+    }
+    std::memcpy(out, prod.data(), sizeof(uint16_t) * t);
+}
+
+// Minimal polynomial of f (t coefficients) over GF(2^m): g monic of degree t with
+// f^t + sum_{j<t} g_j f^j = 0.  Returns false when the powers are dependent.
+static bool minimal_poly(int pidx, const uint16_t *f, uint16_t *g)
+{
+    const int t = kParams[pidx].t;
+    HostField F(kParams[pidx].m);
+    std::vector<ExtTap> taps;
+    ext_taps(pidx, taps);
+    // columns: f^0 .. f^t, each t coefficients; matrix M[row r][col j] = (f^j)_r
+    std::vector<uint16_t> M((size_t)t * (t + 1), 0);
+    std::vector<uint16_t> cur(t, 0), nxt(t);
+    cur[0] = 1;
+    for (int j = 0; j <= t; j++) {
+        for (int r = 0; r < t; r++) M[(size_t)r * (t + 1) + j] = cur[r];
+        if (j < t) {
+            ext_mul(F, t, taps, cur.data(), f, nxt.data());
+            cur.swap(nxt);
+        }
+    }
+    // Solve sum_j g_j col_j = col_t by Gauss-Jordan (char 2: minus == plus).
+    for (int c = 0; c < t; c++) {
+        int piv = -1;
+        for (int r = c; r < t; r++)
+            if (M[(size_t)r * (t + 1) + c]) { piv = r; break; }
+        if (piv < 0) return false;
+        if (piv != c)
+            for (int k = 0; k <= t; k++) std::swap(M[(size_t)piv * (t + 1) + k], M[(size_t)c * (t + 1) + k]);
+        uint16_t inv = F.inv(M[(size_t)c * (t + 1) + c]);
+        for (int k = 0; k <= t; k++) M[(size_t)c * (t + 1) + k] = F.mul(M[(size_t)c * (t + 1) + k], inv);
+        for (int r = 0; r < t; r++) {
+            if (r == c) continue;
+            uint16_t fac = M[(size_t)r * (t + 1) + c];
+            if (!fac) continue;
+            for (int k = c; k <= t; k++) M[(size_t)r * (t + 1) + k] ^= F.mul(fac, M[(size_t)c * (t + 1) + k]);
+        }
+    }
+    for (int r = 0; r < t; r++) g[r] = M[(size_t)r * (t + 1) + t];
+    return true;
+}
+
+// ---- Benes routing -------------------------------------------------------------------
+
+static inline void set_cond_bit(uint8_t *cond, int m, int layer, int p, int e)
+{
+    // switch index of the pair (p, p + 2^e) with bit e of p clear
+    const int idx = ((p >> (e + 1)) << e) | (p & ((1 << e) - 1));
+    const size_t bit = (size_t)layer * ((size_t)1 << (m - 1)) + (size_t)idx;
+    cond[bit >> 3] |= (uint8_t)(1u << (bit & 7));
+}
+
+// Routes pi (size 2^w, output i receives input pi[i]) through the sub-network at depth d
+// whose global positions are (q << d) | c.
+static void route(int m, int w, int d, int c, const std::vector<int> &pi, uint8_t *cond)
+{
+    const int n = 1 << w;
+    const int first = d, last = 2 * m - 2 - d;
+    if (w == 1) {
+        if (pi[0] == 1) set_cond_bit(cond, m, first, c, d);
+        return;
+    }
+    std::vector<int> where(n);  // where[v] = output position of value v
+    for (int i = 0; i < n; i++) where[pi[i]] = i;
+    std::vector<int> color(n, -1);  // sub-network of each value
+    for (int start = 0; start < n; start++) {
+        if (color[start] >= 0) continue;
+        int v = start, col = 0;
+        while (color[v] < 0) {
+            color[v] = col;
+            int u = pi[where[v] ^ 1];  // shares v's output pair
+            color[u] = col ^ 1;
+            v = u ^ 1;                 // shares u's input pair
+        }
+    }
+    std::vector<int> sub0(n / 2), sub1(n / 2);
+    for (int a = 0; a < n / 2; a++)
+        if (color[2 * a] == 1) set_cond_bit(cond, m, first, ((2 * a) << d) | c, d);
+    for (int b = 0; b < n / 2; b++) {
+        int v0 = pi[2 * b], v1 = pi[2 * b + 1];
+        if (color[v0] == 1) set_cond_bit(cond, m, last, ((2 * b) << d) | c, d);
+        int in0 = color[v0] == 0 ? v0 : v1;
+        int in1 = color[v0] == 0 ? v1 : v0;
+        sub0[b] = in0 >> 1;
+        sub1[b] = in1 >> 1;
+    }
+    route(m, w - 1, d + 1, c, sub0, cond);
+    route(m, w - 1, d + 1, c | (1 << d), sub1, cond);
+}
+
+// Condition bits (cond_bytes) for permutation pi of 2^m; verified by re-applying them.
+static bool controlbits(int m, const std::vector<int> &pi, uint8_t *cond, size_t cond_bytes)
+{
+    std::memset(cond, 0, cond_bytes);
+    route(m, m, 0, 0, pi, cond);
+    std::vector<uint16_t> check((size_t)1 << m);
+    for (size_t i = 0; i < check.size(); i++) check[i] = (uint16_t)i;
+    benes_apply(m, check.data(), cond);
+    for (size_t i = 0; i < check.size(); i++)
+        if (check[i] != pi[i]) return false;
+    return true;
+}
+
+// ---- encryption ------------------------------------------------------------------------
+
+struct Encryptor {
+    int pidx = -1, mt = 0, words = 0;
+    std::vector<uint64_t> cols;  // n columns of H, mt bits each
+    std::vector<uint64_t> ainv;  // mt rows of A^-1, mt bits each
+};
+
+// Builds H (rows i*m + k: bit k of L_j^i / g(L_j)) and A^-1.  False when A is singular.
+static bool encryptor_init(int pidx, const uint16_t *g, const uint16_t *L, Encryptor &E)
+{
+    const ParamDesc &p = kParams[pidx];
+    HostField F(p.m);
+    const int mt = p.m * p.t, words = (mt + 63) / 64;
+    E.pidx = pidx;
+    E.mt = mt;
+    E.words = words;
+    E.cols.assign((size_t)p.n * words, 0);
+    for (int j = 0; j < p.n; j++) {
+        uint16_t v = F.inv(F.eval(g, p.t, L[j]));
+        uint64_t *col = &E.cols[(size_t)j * words];
+        for (int i = 0; i < p.t; i++) {
+            for (int k = 0; k < p.m; k++) {
+                int r = i * p.m + k;
+                col[r >> 6] |= (uint64_t)((v >> k) & 1u) << (r & 63);
+            }
+            v = F.mul(v, L[j]);
+        }
+    }
+    // [A | I] as rows of 2*words words; A[r][c] = bit r of column c.
+    const int w2 = 2 * words;
+    std::vector<uint64_t> aug((size_t)mt * w2, 0);
+    for (int c = 0; c < mt; c++) {
+        const uint64_t *col = &E.cols[(size_t)c * words];
+        for (int r = 0; r < mt; r++)
+            if ((col[r >> 6] >> (r & 63)) & 1u) aug[(size_t)r * w2 + (c >> 6)] |= 1ull << (c & 63);
+    }
+    for (int r = 0; r < mt; r++) aug[(size_t)r * w2 + words + (r >> 6)] |= 1ull << (r & 63);
+    for (int c = 0; c < mt; c++) {
+        int piv = -1;
+        for (int r = c; r < mt; r++)
+            if ((aug[(size_t)r * w2 + (c >> 6)] >> (c & 63)) & 1u) { piv = r; break; }
+        if (piv < 0) return false;
+        if (piv != c)
+            for (int k = 0; k < w2; k++) std::swap(aug[(size_t)piv * w2 + k], aug[(size_t)c * w2 + k]);
+        const uint64_t *prow = &aug[(size_t)c * w2];
+        for (int r = 0; r < mt; r++) {
+            if (r == c) continue;
+            uint64_t *row = &aug[(size_t)r * w2];
+            if ((row[c >> 6] >> (c & 63)) & 1u)
+                for (int k = c >> 6; k < w2; k++) row[k] ^= prow[k];
+        }
+    }
+    E.ainv.assign((size_t)mt * words, 0);
+    for (int r = 0; r < mt; r++)
+        std::memcpy(&E.ainv[(size_t)r * words], &aug[(size_t)r * w2 + words], sizeof(uint64_t) * words);
+    return true;
+}
+
+static void encrypt_one(const Encryptor &E, Rng &rng, int weight, uint8_t *ct, uint8_t *e)
+{
+    const ParamDesc &p = kParams[E.pidx];
+    std::vector<int> pos(p.n);
+    for (int i = 0; i < p.n; i++) pos[i] = i;
+    for (int i = 0; i < weight; i++) std::swap(pos[i], pos[i + rng.below((uint32_t)(p.n - i))]);
+    std::memset(e, 0, (size_t)p.n / 8);
+    std::vector<uint64_t> y(E.words, 0);
+    for (int i = 0; i < weight; i++) {
+        int j = pos[i];
+        e[j >> 3] |= (uint8_t)(1u << (j & 7));
+        const uint64_t *col = &E.cols[(size_t)j * E.words];
+        for (int k = 0; k < E.words; k++) y[k] ^= col[k];
+    }
+    std::memset(ct, 0, (size_t)p.synd_bytes);
+    for (int r = 0; r < E.mt; r++) {
+        const uint64_t *row = &E.ainv[(size_t)r * E.words];
+        uint64_t acc = 0;
+        for (int k = 0; k < E.words; k++) acc ^= row[k] & y[k];
+        if (__builtin_parityll(acc)) ct[r >> 3] |= (uint8_t)(1u << (r & 7));
+    }
+}
+
+// ---- key pairs -------------------------------------------------------------------------
+
+// One attempt per seed value; returns false when this seed gives no usable key.
+static bool keypair_attempt(int pidx, uint64_t seed, uint8_t *sk)
+{
+    const ParamDesc &p = kParams[pidx];
+    HostField F(p.m);
+    Rng rng(seed ^ ((uint64_t)p.id << 32));
+    std::vector<uint16_t> f(p.t), g(p.t + 1);
+    for (int i = 0; i < p.t; i++) f[i] = (uint16_t)(rng.next() & F.mask);
+    if (!minimal_poly(pidx, f.data(), g.data())) return false;
+    g[p.t] = 1;
+    const int q = 1 << p.m;
+    std::vector<int> pi(q);
+    for (int i = 0; i < q; i++) pi[i] = i;
+    for (int i = q - 1; i > 0; i--) std::swap(pi[i], pi[rng.below((uint32_t)(i + 1))]);
+    std::vector<uint8_t> cond(p.cond_bytes);
+    if (!controlbits(p.m, pi, cond.data(), cond.size())) return false;
+    std::vector<uint16_t> L(p.n);
+    support_gen(pidx, cond.data(), L.data());
+    Encryptor E;
+    if (!encryptor_init(pidx, g.data(), L.data(), E)) return false;
+
+    uint8_t *w = sk;
+    for (int i = 0; i < 32; i++) w[i] = (uint8_t)(rng.next() & 0xFF);
+    w += 32;
+    const uint8_t pivots[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0, 0, 0, 0};  // operations.c:101
+    std::memcpy(w, pivots, 8);
+    w += 8;
+    for (int i = 0; i < p.t; i++) { w[2 * i] = (uint8_t)(g[i] & 0xFF); w[2 * i + 1] = (uint8_t)(g[i] >> 8); }
+    w += 2 * p.t;
+    std::memcpy(w, cond.data(), cond.size());
+    w += cond.size();
+    for (int i = 0; i < p.n / 8; i++) w[i] = (uint8_t)(rng.next() & 0xFF);
+    return true;
+}
+
+}  // namespace cmd
+
+using namespace cmd;
+
+extern "C" int cmd_gen_keypair(int32_t param_id, uint64_t seed, uint8_t *sk)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return CMD_ERR_PARAM;
+    if (!sk) return CMD_ERR_ARG;
+    // Deterministic: the first seed value s, s+2^40, s+2*2^40, ... that yields a key.
+    for (uint64_t k = 0; k < 64; k++)
+        if (keypair_attempt(pidx, seed + (k << 40), sk)) return CMD_OK;
+    return CMD_ERR_SINGULAR;
+}
+
+extern "C" int cmd_gen_encrypt(int32_t param_id, const uint8_t *sk, size_t sk_len, uint64_t seed,
+                               size_t count, int32_t weight, uint8_t *ct, uint8_t *e)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return CMD_ERR_PARAM;
+    const ParamDesc &p = kParams[pidx];
+    if (!sk || (!ct && count)) return CMD_ERR_ARG;
+    if (weight < 0) weight = p.t;
+    if (weight > p.t) return CMD_ERR_ARG;
+    ExpandedKey K;
+    int rc = expand_key(pidx, sk, sk_len, K);
+    if (rc) return rc;
+    Encryptor E;
+    if (!encryptor_init(pidx, K.g.data(), K.L.data(), E)) return CMD_ERR_SINGULAR;
+    Rng rng(seed ^ 0xC1F3A5E7D2B49680ull ^ ((uint64_t)p.id << 20));
+    std::vector<uint8_t> tmp_e((size_t)p.n / 8);
+    for (size_t i = 0; i < count; i++)
+        encrypt_one(E, rng, weight, ct + i * (size_t)p.synd_bytes, e ? e + i * (size_t)(p.n / 8) : tmp_e.data());
+    return CMD_OK;
+}
+
+extern "C" int cmd_controlbits(int32_t m, const int16_t *pi, uint8_t *cond)
+{
+    if ((m != 12 && m != 13) || !pi || !cond) return CMD_ERR_ARG;
+    const int q = 1 << m;
+    std::vector<int> perm(q);
+    std::vector<char> seen(q, 0);
+    for (int i = 0; i < q; i++) {
+        int v = pi[i];
+        if (v < 0 || v >= q || seen[v]) return CMD_ERR_ARG;  // not a permutation
+        seen[v] = 1;
+        perm[i] = v;
+    }
+    const size_t bytes = (size_t)(2 * m - 1) * ((size_t)1 << (m - 4));
+    return controlbits(m, perm, cond, bytes) ? CMD_OK : CMD_ERR_ARG;
+}
+
+extern "C" int cmd_gen_keys(int32_t param_id, uint64_t base_seed, size_t n_keys, uint8_t *sk_out, int threads)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return CMD_ERR_PARAM;
+    if (!sk_out && n_keys) return CMD_ERR_ARG;
+    const size_t skb = (size_t)kParams[pidx].sk_bytes;
+    if (threads < 1) threads = 1;
+    std::atomic<size_t> next{0};
+    std::atomic<int> status{CMD_OK};
+    auto work = [&]() {
+        for (;;) {
+            size_t k = next.fetch_add(1);
+            if (k >= n_keys) return;
+            int rc = cmd_gen_keypair(param_id, base_seed + k, sk_out + k * skb);
+            if (rc) status.store(rc);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < threads; i++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    return status.load();
+}

@@ -1,0 +1,62 @@
+// Launch interface between the host engine and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace cmd {
+
+// Device view of one resident key.
+struct KeyDev {
+    const uint16_t *table;  // sb x 2t: L_i^j / g(L_i)^2
+    const uint16_t *L;      // n support elements
+    const uint32_t *lbits;  // m x lbits_stride bit planes of L (32 positions per word)
+    int32_t pidx;
+    int32_t lstride;        // words per bit-plane row
+};
+
+// ---- plain per-stage pipeline (one key per launch group) ----------------------------
+// Items k < count use ciphertext ct + idx[k]*ct_stride and write e + idx[k]*e_stride;
+// idx == nullptr means identity.  s: count x 2t, loc: count x (t+1).
+hipError_t launch_plain_synd(int pidx, const KeyDev &key, int count, const int32_t *idx,
+                             const uint8_t *ct, size_t ct_stride, uint16_t *s, hipStream_t st);
+hipError_t launch_plain_bm(int pidx, int count, const uint16_t *s, uint16_t *loc, hipStream_t st);
+hipError_t launch_plain_chien(int pidx, const KeyDev &key, int count, const int32_t *idx,
+                              const uint16_t *loc, uint8_t *e, size_t e_stride, hipStream_t st);
+
+// ---- fused persistent pipeline -------------------------------------------------------
+// One work item of the fused kernel: up to 32 ciphertexts of one parameter set.
+struct FusedTile {
+    int32_t pidx;   // parameter set
+    int32_t first;  // offset into the item list
+    int32_t count;  // items in this tile (<= tile width)
+    int32_t pad;
+};
+
+struct FusedArgs {
+    const FusedTile *tiles;
+    int32_t n_tiles;
+    int32_t pad0;
+    const int32_t *items;      // item index (into ct/e) per tile slot
+    const int32_t *item_key;   // key slot per tile slot
+    const KeyDev *keys;        // resident key table (device)
+    const uint8_t *ct;
+    uint8_t *e;
+    uint64_t ct_stride, e_stride;
+    uint32_t *tile_counter;    // work-queue head, zeroed before each launch
+};
+
+struct FusedSchedule {
+    int tile_cts;         // 8, 16 or 32
+    int waves_per_group;  // 1..8
+    int waves_per_ct;     // 1, 2 or 4
+    int groups_per_cu;    // persistent workgroups per CU
+};
+
+// Variant of the fused kernel compiled for a schedule, or false if not built.
+bool fused_supported(const FusedSchedule &s);
+hipError_t launch_fused(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t st);
+
+}  // namespace cmd

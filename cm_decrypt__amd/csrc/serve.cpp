@@ -1,0 +1,429 @@
+// Continuous-batching decryption server over the fused kernel.
+//
+// Requests (key id + ciphertext) queue on the host.  A worker thread keeps up to
+// max_inflight launches on the device, each on its own stream with its own pinned staging
+// and device buffers: whenever a launch slot is free and work is pending, the pending
+// requests (any mix of parameter sets and resident keys) are planned into fused-kernel
+// tiles and launched, so requests that arrive while others run on the GPU join the next
+// launch and overlap with the running ones.  Finished launches are retired by polling
+// their completion events; each request's error vector is copied out and its latency
+// (submit -> result available) recorded.
+//
+// Admission: keys are admitted by the context's HBM budget when loaded; a key with
+// requests in flight cannot be unloaded; the in-flight I/O buffers are allocated up front
+// within io_budget_bytes, and submit() rejects with CMD_ERR_FULL once queue_capacity
+// requests are pending.  Every request's output lands in its own slot of its launch, so a
+// result never depends on what else shares the launch.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "engine.h"
+
+using namespace cmd;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_us()
+{
+    return std::chrono::duration<double, std::micro>(Clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+    uint64_t id = 0;
+    int32_t key = -1;
+    int pidx = -1;
+    int state = 0;  // 0 pending, 1 in flight, 2 done
+    double t_submit = 0, t_done = 0;
+    uint8_t ct[kMaxSyndBytes];
+    uint8_t e[kMaxEBytes];
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    uint8_t *h_ct = nullptr, *h_e = nullptr;  // pinned staging
+    char *h_plan = nullptr;                   // pinned plan staging
+    DevBuf dev;                               // d_ct | d_e | plan
+    std::vector<Req *> reqs;
+    bool busy = false;
+    double t_launch = 0;
+};
+
+}  // namespace
+
+struct cmd_server {
+    cmd_ctx *ctx = nullptr;
+    cmd_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req *> pending;
+    std::unordered_map<uint64_t, Req *> reqs;
+    std::vector<Slot> slots;
+    std::thread worker;
+    bool stop = false;
+    int worker_rc = CMD_OK;
+    uint64_t next_id = 1;
+    // statistics
+    uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_items = 0;
+    std::vector<double> lat_us;
+    double busy_since = -1, busy_us = 0;
+    int busy_slots = 0;
+    // per-slot plan sizing
+    size_t plan_bytes = 0;
+};
+
+namespace {
+
+constexpr size_t kCtStride = kMaxSyndBytes;
+constexpr size_t kEStride = kMaxEBytes;
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Launches the requests in `batch` on slot s.  Called with srv->mu held.
+int launch_slot(cmd_server *srv, Slot &s, std::vector<Req *> &batch)
+{
+    cmd_ctx *ctx = srv->ctx;
+    const size_t count = batch.size();
+    std::vector<int32_t> key_ids(count);
+    for (size_t i = 0; i < count; i++) key_ids[i] = batch[i]->key;
+    std::vector<FusedTile> tiles;
+    std::vector<int32_t> items, item_key;
+    int launch_tile = 32;
+    FusedSchedule sched;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->lock);
+        plan_fused(ctx, ctx->sched, count, key_ids.data(), tiles, items, item_key, &launch_tile);
+        sched = launch_schedule(ctx->sched, tiles, launch_tile);
+    }
+    for (size_t i = 0; i < count; i++) std::memcpy(s.h_ct + i * kCtStride, batch[i]->ct, kParams[batch[i]->pidx].synd_bytes);
+
+    const size_t mb = (size_t)srv->cfg.max_batch;
+    char *base = (char *)s.dev.ptr;
+    uint8_t *d_ct = (uint8_t *)base;
+    uint8_t *d_e = (uint8_t *)(base + align256(mb * kCtStride));
+    char *d_plan = base + align256(mb * kCtStride) + align256(mb * kEStride);
+    const size_t off_keys = align256(mb * 4), off_tiles = off_keys + align256(mb * 4);
+    const size_t off_ctr = off_tiles + align256(mb * sizeof(FusedTile));
+    std::memcpy(s.h_plan, items.data(), items.size() * 4);
+    std::memcpy(s.h_plan + off_keys, item_key.data(), item_key.size() * 4);
+    std::memcpy(s.h_plan + off_tiles, tiles.data(), tiles.size() * sizeof(FusedTile));
+
+    CMD_HIP(hipMemcpyAsync(d_ct, s.h_ct, count * kCtStride, hipMemcpyHostToDevice, s.stream));
+    CMD_HIP(hipMemcpyAsync(d_plan, s.h_plan, off_ctr, hipMemcpyHostToDevice, s.stream));
+    FusedArgs a;
+    a.tiles = (const FusedTile *)(d_plan + off_tiles);
+    a.n_tiles = (int32_t)tiles.size();
+    a.pad0 = 0;
+    a.items = (const int32_t *)d_plan;
+    a.item_key = (const int32_t *)(d_plan + off_keys);
+    a.keys = ctx->d_keys;
+    a.ct = d_ct;
+    a.e = d_e;
+    a.ct_stride = kCtStride;
+    a.e_stride = kEStride;
+    a.tile_counter = (uint32_t *)(d_plan + off_ctr);
+    CMD_HIP(launch_zero_u32(a.tile_counter, 1, s.stream));
+    CMD_HIP(launch_fused(a, sched, ctx->n_cu, s.stream));
+    CMD_HIP(hipMemcpyAsync(s.h_e, d_e, count * kEStride, hipMemcpyDeviceToHost, s.stream));
+    CMD_HIP(hipEventRecord(s.done, s.stream));
+
+    s.reqs.swap(batch);
+    for (Req *r : s.reqs) r->state = 1;
+    s.busy = true;
+    s.t_launch = now_us();
+    if (srv->busy_slots++ == 0) srv->busy_since = s.t_launch;
+    srv->launches++;
+    srv->launched_items += count;
+    return CMD_OK;
+}
+
+void retire_slot(cmd_server *srv, Slot &s)
+{
+    const double t = now_us();
+    {
+        std::lock_guard<std::recursive_mutex> g(srv->ctx->lock);
+        for (Req *r : s.reqs) srv->ctx->keys[r->key].inflight--;
+    }
+    for (size_t i = 0; i < s.reqs.size(); i++) {
+        Req *r = s.reqs[i];
+        std::memcpy(r->e, s.h_e + i * kEStride, kParams[r->pidx].n / 8);
+        r->state = 2;
+        r->t_done = t;
+        srv->lat_us.push_back(t - r->t_submit);
+        srv->completed++;
+    }
+    s.reqs.clear();
+    s.busy = false;
+    if (--srv->busy_slots == 0) {
+        srv->busy_us += t - srv->busy_since;
+        srv->busy_since = -1;
+    }
+}
+
+// Marks requests finished without a result (after a device error); waiters get worker_rc.
+void fail_requests(cmd_server *srv, std::vector<Req *> &batch)
+{
+    std::lock_guard<std::recursive_mutex> g(srv->ctx->lock);
+    for (Req *r : batch) {
+        srv->ctx->keys[r->key].inflight--;
+        r->state = 2;
+        r->t_done = now_us();
+    }
+    batch.clear();
+}
+
+void worker_main(cmd_server *srv)
+{
+    (void)hipSetDevice(srv->ctx->device);
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        bool progressed = false;
+        for (Slot &s : srv->slots) {
+            if (!s.busy) continue;
+            hipError_t q = hipEventQuery(s.done);
+            if (q == hipSuccess) {
+                retire_slot(srv, s);
+                progressed = true;
+            } else if (q != hipErrorNotReady) {
+                srv->worker_rc = hip_fail(q, "server launch");
+                retire_slot(srv, s);  // results are invalid; waiters see the error code
+                progressed = true;
+            }
+        }
+        if (progressed) srv->cv_done.notify_all();
+
+        if (!srv->pending.empty() && srv->worker_rc == CMD_OK) {
+            Slot *free_slot = nullptr;
+            for (Slot &s : srv->slots)
+                if (!s.busy) { free_slot = &s; break; }
+            const double waited = now_us() - srv->pending.front()->t_submit;
+            const bool ready = (int)srv->pending.size() >= srv->cfg.min_batch || waited >= srv->cfg.max_wait_us ||
+                               srv->busy_slots == 0 || srv->stop;
+            if (free_slot && ready) {
+                std::vector<Req *> batch;
+                while (!srv->pending.empty() && (int)batch.size() < srv->cfg.max_batch) {
+                    batch.push_back(srv->pending.front());
+                    srv->pending.pop_front();
+                }
+                int rc = launch_slot(srv, *free_slot, batch);
+                if (rc) {
+                    srv->worker_rc = rc;
+                    fail_requests(srv, batch);
+                    srv->cv_done.notify_all();
+                }
+                progressed = true;
+            }
+        }
+        if (srv->worker_rc != CMD_OK && !srv->pending.empty()) {
+            std::vector<Req *> rest(srv->pending.begin(), srv->pending.end());
+            srv->pending.clear();
+            fail_requests(srv, rest);
+            srv->cv_done.notify_all();
+        }
+        if (srv->stop && srv->pending.empty() && srv->busy_slots == 0) break;
+        if (!progressed) {
+            // poll launches in flight often; otherwise sleep until a submit arrives
+            if (srv->busy_slots > 0 || !srv->pending.empty())
+                srv->cv_work.wait_for(lk, std::chrono::microseconds(10));
+            else
+                srv->cv_work.wait(lk);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd_server **out)
+{
+    if (!ctx || !cfg || !out) return CMD_ERR_ARG;
+    *out = nullptr;
+    cmd_server_config c = *cfg;
+    if (c.max_batch < 1) c.max_batch = 4096;
+    if (c.max_inflight < 1) c.max_inflight = 2;
+    if (c.max_inflight > 8) c.max_inflight = 8;
+    if (c.queue_capacity < 1) c.queue_capacity = 1 << 20;
+    if (c.min_batch < 1) c.min_batch = 1;
+    if (c.max_wait_us < 0) c.max_wait_us = 0;
+    const size_t mb = (size_t)c.max_batch;
+    const size_t plan = align256(mb * 4) * 2 + align256(mb * sizeof(FusedTile)) + 256;
+    const size_t per_slot = align256(mb * kCtStride) + align256(mb * kEStride) + plan;
+    if (c.io_budget_bytes && per_slot * (size_t)c.max_inflight > c.io_budget_bytes)
+        return set_error(CMD_ERR_BUDGET, "%d launch slots of %zu bytes exceed the I/O budget of %llu bytes",
+                         c.max_inflight, per_slot, (unsigned long long)c.io_budget_bytes);
+
+    std::unique_ptr<cmd_server> srv(new cmd_server());
+    srv->ctx = ctx;
+    srv->cfg = c;
+    srv->plan_bytes = plan;
+    srv->slots.resize(c.max_inflight);
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    int rc = CMD_OK;
+    for (Slot &s : srv->slots) {
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void **)&s.h_ct, mb * kCtStride, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&s.h_e, mb * kEStride, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&s.h_plan, plan, hipHostMallocDefault) != hipSuccess) {
+            rc = set_error(CMD_ERR_HIP, "server slot allocation failed");
+            break;
+        }
+        rc = dev_alloc(ctx, s.dev, per_slot);
+        if (rc) break;
+    }
+    if (rc) {
+        for (Slot &s : srv->slots) {
+            if (s.stream) (void)hipStreamDestroy(s.stream);
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.h_ct) (void)hipHostFree(s.h_ct);
+            if (s.h_e) (void)hipHostFree(s.h_e);
+            if (s.h_plan) (void)hipHostFree(s.h_plan);
+            dev_free(ctx, s.dev);
+        }
+        return rc;
+    }
+    cmd_server *raw = srv.get();
+    srv->worker = std::thread([raw]() { worker_main(raw); });
+    *out = srv.release();
+    return CMD_OK;
+}
+
+extern "C" int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *ct, uint64_t *req)
+{
+    if (!s || !ct || !req) return CMD_ERR_ARG;
+    int pidx;
+    {
+        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+        if (key_id < 0 || (size_t)key_id >= s->ctx->keys.size() || !s->ctx->keys[key_id].used)
+            return set_error(CMD_ERR_KEY, "key id %d is not loaded", (int)key_id);
+        pidx = s->ctx->keys[key_id].pidx;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return set_error(CMD_ERR_ARG, "server is shutting down");
+    if ((int)s->pending.size() >= s->cfg.queue_capacity) {
+        s->rejected++;
+        return set_error(CMD_ERR_FULL, "%d requests pending", (int)s->pending.size());
+    }
+    Req *r = new (std::nothrow) Req();
+    if (!r) return CMD_ERR_NOMEM;
+    r->id = s->next_id++;
+    r->key = key_id;
+    r->pidx = pidx;
+    std::memcpy(r->ct, ct, kParams[pidx].synd_bytes);
+    r->t_submit = now_us();
+    {
+        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+        s->ctx->keys[key_id].inflight++;
+    }
+    s->reqs[r->id] = r;
+    s->pending.push_back(r);
+    s->submitted++;
+    *req = r->id;
+    s->cv_work.notify_one();
+    return CMD_OK;
+}
+
+static int take_result(cmd_server *s, uint64_t id, uint8_t *e_out)
+{
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+    Req *r = it->second;
+    if (r->state != 2) return CMD_ERR_PENDING;
+    if (s->worker_rc) return s->worker_rc;
+    if (e_out) std::memcpy(e_out, r->e, kParams[r->pidx].n / 8);
+    s->reqs.erase(it);
+    delete r;
+    return CMD_OK;
+}
+
+extern "C" int cmd_server_poll(cmd_server *s, uint64_t req, uint8_t *e_out)
+{
+    if (!s) return CMD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return take_result(s, req, e_out);
+}
+
+extern "C" int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *e_out, double timeout_us)
+{
+    if (!s) return CMD_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    const auto deadline = Clock::now() + std::chrono::microseconds((long long)(timeout_us < 0 ? 0 : timeout_us));
+    for (;;) {
+        int rc = take_result(s, req, e_out);
+        if (rc != CMD_ERR_PENDING) return rc;
+        if (timeout_us < 0) {
+            s->cv_done.wait(lk);
+        } else if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+            rc = take_result(s, req, e_out);
+            return rc == CMD_ERR_PENDING ? CMD_ERR_TIMEOUT : rc;
+        }
+    }
+}
+
+extern "C" int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out)
+{
+    if (!s || !out) return CMD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->rejected = s->rejected;
+    out->launches = s->launches;
+    out->mean_batch = s->launches ? (double)s->launched_items / (double)s->launches : 0.0;
+    std::vector<double> v = s->lat_us;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double q) {
+        if (v.empty()) return 0.0;
+        size_t i = (size_t)(q * (double)(v.size() - 1) + 0.5);
+        return v[std::min(i, v.size() - 1)];
+    };
+    out->p50_us = pct(0.50);
+    out->p99_us = pct(0.99);
+    out->max_us = v.empty() ? 0.0 : v.back();
+    double busy = s->busy_us;
+    if (s->busy_since >= 0) busy += now_us() - s->busy_since;
+    out->busy_s = busy * 1e-6;
+    return CMD_OK;
+}
+
+extern "C" int cmd_server_reset_stats(cmd_server *s)
+{
+    if (!s) return CMD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->submitted = s->completed = s->rejected = s->launches = s->launched_items = 0;
+    s->lat_us.clear();
+    s->busy_us = 0;
+    if (s->busy_since >= 0) s->busy_since = now_us();
+    return CMD_OK;
+}
+
+extern "C" int cmd_server_destroy(cmd_server *s)
+{
+    if (!s) return CMD_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    (void)hipSetDevice(s->ctx->device);
+    for (Slot &sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.h_ct) (void)hipHostFree(sl.h_ct);
+        if (sl.h_e) (void)hipHostFree(sl.h_e);
+        if (sl.h_plan) (void)hipHostFree(sl.h_plan);
+        dev_free(s->ctx, sl.dev);
+    }
+    for (auto &kv : s->reqs) delete kv.second;  // every request was retired by the worker
+    delete s;
+    return CMD_OK;
+}

@@ -1,0 +1,227 @@
+/*
+ * cm_decrypt.h -- C ABI of the MI355X Classic McEliece batched decryption engine.
+ *
+ * Plain C types only (pointers, sizes, integers).  All buffers are caller-owned.
+ * Every function returns CMD_OK (0) or a negative error code; the library never
+ * calls exit().  A context may be used from several host threads (calls are
+ * serialised by an internal lock); a server object is thread-safe.
+ *
+ * Reference interface replaced by each entry point (paths relative to the
+ * reference repository):
+ *   - the file contract of the drivers: ct_<p>.bin (KATNUM x CIPHERTEXTBYTES, raw),
+ *     sk_<p>.bin (SECRETKEYBYTES), errorstream0_<p>.bin (text)
+ *     (CPU/param/param6960119/src/util.c:30-58, decrypt.c:55-66, 138);
+ *   - the C functions initialisation / compute_inverses / synd / bm / root /
+ *     support_gen (CPU/include/common/root.h:10-25, bm.h:13, GPU_Baseline/include/common/root.h:13-28);
+ *   - the GPU drivers' batch loop decrypt()/decrypt_all()
+ *     (GPU_Baseline/param/param8192128/src/cuda/Decrypt.cu:177-324,
+ *      GPU_Baseline/param/param6960119/src/cuda/decrypt.cu:259-434,
+ *      GPU_Optimised/param/param8192128/src/cuda/Decrypt.cu:220-314).
+ */
+#ifndef CM_DECRYPT_H
+#define CM_DECRYPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------------------- */
+#define CMD_OK             0
+#define CMD_ERR_FILE      (-1)   /* KAT_FILE_OPEN_ERROR  (CPU/param/param6960119/include/common.h:11) */
+#define CMD_ERR_CRYPTO    (-4)   /* KAT_CRYPTO_FAILURE   (common.h:12): bad KATNUM / bad key bytes */
+#define CMD_ERR_PARAM     (-10)  /* unknown parameter set id */
+#define CMD_ERR_ARG       (-11)  /* invalid argument (null pointer, stride too small, ...) */
+#define CMD_ERR_HIP       (-12)  /* HIP runtime error (message in cmd_last_error) */
+#define CMD_ERR_NODEV     (-13)  /* no usable GPU */
+#define CMD_ERR_BUDGET    (-14)  /* HBM budget of the context would be exceeded */
+#define CMD_ERR_KEY       (-15)  /* unknown or unloaded key id, or key of another parameter set */
+#define CMD_ERR_FULL      (-16)  /* server queue full */
+#define CMD_ERR_NOMEM     (-17)  /* host allocation failed */
+#define CMD_ERR_SINGULAR  (-18)  /* generator: key has no systematic form */
+#define CMD_ERR_PENDING   (-19)  /* server: request not finished yet */
+#define CMD_ERR_TIMEOUT   (-20)  /* server: wait timed out */
+
+/* Text for an error code; cmd_last_error() gives the calling thread's last detail. */
+const char *cmd_strerror(int code);
+const char *cmd_last_error(void);
+
+/* ---- parameter sets ---------------------------------------------------------------- */
+/* ids: 348864, 460896, 6688128, 6960119, 8192128 (CPU/param/param<p>/include/common.h) */
+typedef struct {
+    int32_t id, m, n, t;
+    int32_t ct_bytes;    /* crypto_kem_CIPHERTEXTBYTES == SYND_BYTES */
+    int32_t sb;          /* syndrome input bits, 8 * SYND_BYTES */
+    int32_t sk_bytes;    /* crypto_kem_SECRETKEYBYTES */
+    int32_t cond_bytes;  /* COND_BYTES */
+    int32_t e_bytes;     /* n / 8 */
+} cmd_param_info;
+
+int cmd_param_get(int32_t param_id, cmd_param_info *out);
+
+/* ---- host-only key handling (no GPU needed) ------------------------------------------ */
+
+/* support_gen(gf *s, const unsigned char *c): CPU/src/common/root_13.c:181-211,
+ * CPU/param/param348864/src/root.c:336-385.  L receives n elements. */
+int cmd_support_gen(int32_t param_id, const uint8_t *cond, uint16_t *L);
+
+/* initialisation() key part + compute_inverses(): CPU/param/param6960119/src/util.c:60-100.
+ * g: t+1 coefficients (monic), L: n, table: sb x 2t (row-major); any may be NULL. */
+int cmd_key_expand_host(int32_t param_id, const uint8_t *sk, size_t sk_len,
+                        uint16_t *g, uint16_t *L, uint16_t *table);
+
+/* write_error_positions (CPU/param/param6960119/src/decrypt.c:55-66) for `count` error
+ * vectors of n/8 bytes, e_stride bytes apart: " %d" per set bit ascending, '\n' per vector.
+ * Writes at most cap bytes to buf; *written receives the full length (call with cap 0 to size). */
+int cmd_errorstream_format(int32_t param_id, const uint8_t *e, size_t e_stride, size_t count,
+                           char *buf, size_t cap, size_t *written);
+
+/* Writes (append != 0: appends) the errorstream file; the reference drivers reset it first
+ * (decrypt.c:138) or append per ciphertext (CPU/param/param8192128/src/Decrypt.c:161). */
+int cmd_errorstream_write(int32_t param_id, const char *path, const uint8_t *e, size_t e_stride,
+                          size_t count, int append);
+
+/* ---- seeded test-vector generator (replaces kem/: keypair, gen_e, syndrome) --------------
+ * kem/common/core/operations.c:36-107 and encrypt.c:40-120 define the formats; the
+ * randomness here is this library's own seeded generator, not the NIST DRBG. */
+
+/* Writes sk_bytes to sk: 32 B seed | ff ff ff ff 00 00 00 00 | g (t x 2 B LE) | Benes
+ * condition bits of a seeded permutation | n/8 B s.  g is a seeded monic irreducible
+ * polynomial; keys whose first m*t parity-check columns are singular are redrawn. */
+int cmd_gen_keypair(int32_t param_id, uint64_t seed, uint8_t *sk);
+
+/* `count` ciphertexts for key sk, each with a uniformly random error of exactly `weight`
+ * distinct positions in [0, n) (weight <= t; weight < 0 means t).  ct: count x ct_bytes,
+ * e: count x e_bytes (the error vectors, the expected decoder output for weight == t). */
+int cmd_gen_encrypt(int32_t param_id, const uint8_t *sk, size_t sk_len, uint64_t seed,
+                    size_t count, int32_t weight, uint8_t *ct, uint8_t *e);
+
+/* Benes condition bits for a permutation pi of {0..2^m-1} (controlbitsfrompermutation,
+ * kem/common/core/controlbits.c:180-220): applying the network to the identity yields pi,
+ * and support_gen over these bits gives L[i] = bitrev(pi[i]).  cond: (2m-1)*2^(m-4) bytes. */
+int cmd_controlbits(int32_t m, const int16_t *pi, uint8_t *cond);
+
+/* Many keys at once with `threads` host threads: sk_out holds n_keys x sk_bytes; key k uses
+ * seed base_seed + k (skipping to later seeds like cmd_gen_keypair when needed). */
+int cmd_gen_keys(int32_t param_id, uint64_t base_seed, size_t n_keys, uint8_t *sk_out, int threads);
+
+/* ---- GPU context and resident keys -------------------------------------------------- */
+typedef struct cmd_ctx cmd_ctx;
+
+#define CMD_MODE_PLAIN 0   /* three launches per key group: synd, bm, chien (uint16 GF) */
+#define CMD_MODE_FUSED 1   /* one persistent synd->bm->chien launch for any mix of sets/keys */
+
+/* device: HIP ordinal; hbm_budget_bytes: 0 = 90% of the free device memory. */
+int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx **out);
+int cmd_ctx_destroy(cmd_ctx *ctx);
+/* Device memory charged against the budget (keys + work buffers) and the budget. */
+int cmd_ctx_memory(cmd_ctx *ctx, uint64_t *used, uint64_t *budget);
+
+/* Parses, expands and uploads one key; it stays resident until unloaded.
+ * Replaces initialisation() + compute_inverses() + InitializeC()
+ * (GPU_Baseline/param/param8192128/include/decrypt.h:147-173). */
+int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, size_t sk_len, int32_t *key_id);
+int cmd_key_unload(cmd_ctx *ctx, int32_t key_id);
+/* Device bytes one resident key of this set occupies. */
+int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes);
+
+/* Batch decryption, host buffers (copied in and out, blocking).  Item i uses key key_ids[i]
+ * and its ciphertext at ct + i*ct_stride; its error vector (n/8 bytes, the reference's
+ * build_error_vector, decrypt.c:43-53) goes to e_out + i*e_stride.  Sets and keys may mix. */
+int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
+                const uint8_t *ct, size_t ct_stride, uint8_t *e_out, size_t e_stride);
+
+/* Same on device buffers already resident in HBM, enqueued on `stream` (a hipStream_t,
+ * NULL = the context's stream); key_ids stay on the host.  Returns after enqueueing. */
+int cmd_decrypt_device(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
+                       const void *d_ct, size_t ct_stride, void *d_e, size_t e_stride, void *stream);
+
+/* ---- prepared batches (fixed key assignment, replayed as one hipGraph) --------------- */
+typedef struct cmd_batch cmd_batch;
+
+/* Plans a batch for `mode` (item grouping, schedules, device work list); with
+ * use_graph != 0 the launch sequence is captured into a hipGraph on first run. */
+int cmd_batch_create(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
+                     size_t ct_stride, size_t e_stride, int32_t use_graph, cmd_batch **out);
+int cmd_batch_run(cmd_batch *b, const void *d_ct, void *d_e, void *stream);
+int cmd_batch_destroy(cmd_batch *b);
+
+/* ---- per-stage entry points (reference synd / bm / root on the GPU) ------------------- */
+/* synd (CPU/param/param6960119/src/util.c:102-114) for count ciphertexts of one key;
+ * s_out: count x 2t uint16. */
+int cmd_stage_synd(cmd_ctx *ctx, int32_t key_id, size_t count, const uint8_t *ct,
+                   size_t ct_stride, uint16_t *s_out);
+/* bm (CPU/src/common/bm.c:12-68): s: count x 2t, loc_out: count x (t+1), reversed like bm(). */
+int cmd_stage_bm(cmd_ctx *ctx, int32_t param_id, size_t count, const uint16_t *s, uint16_t *loc_out);
+/* root + build_error_vector (root_13.c:233-240, decrypt.c:43-53): loc: count x (t+1). */
+int cmd_stage_root(cmd_ctx *ctx, int32_t key_id, size_t count, const uint16_t *loc, uint8_t *e_out,
+                   size_t e_stride);
+
+/* ---- schedules (autotuner) ----------------------------------------------------------- */
+/* Fused-kernel schedule per set: ciphertexts per wave-tile (8/16/32), waves per workgroup,
+ * waves per ciphertext in the root stage.  The autotuner measures candidates on the GPU. */
+typedef struct {
+    int32_t tile_cts;        /* ciphertexts per wavefront tile (bit-sliced BM width) */
+    int32_t waves_per_group; /* wavefronts per workgroup */
+    int32_t waves_per_ct;    /* wavefronts sharing one ciphertext's root search */
+    int32_t groups_per_cu;   /* persistent workgroups per compute unit */
+} cmd_schedule;
+
+int cmd_schedule_get(cmd_ctx *ctx, int32_t param_id, cmd_schedule *out);
+int cmd_schedule_set(cmd_ctx *ctx, int32_t param_id, const cmd_schedule *s);
+/* Times each candidate schedule on `count` synthetic items of the set (resident keys
+ * required: key_ids of that set) and keeps the fastest; best_us receives its time. */
+int cmd_autotune(cmd_ctx *ctx, int32_t param_id, size_t count, const int32_t *key_ids,
+                 int32_t reps, cmd_schedule *best, double *best_us);
+/* JSON persistence of the tuned schedules. */
+int cmd_schedule_save(cmd_ctx *ctx, const char *path);
+int cmd_schedule_load(cmd_ctx *ctx, const char *path);
+
+/* ---- serving loop ------------------------------------------------------------------- */
+typedef struct cmd_server cmd_server;
+
+typedef struct {
+    int32_t  max_batch;        /* items per launch (upper bound) */
+    int32_t  max_inflight;     /* launches on the device at once (streams) */
+    int32_t  queue_capacity;   /* pending requests before submit returns CMD_ERR_FULL */
+    int32_t  min_batch;        /* launch as soon as this many are pending ... */
+    double   max_wait_us;      /* ... or the oldest pending request waited this long */
+    uint64_t io_budget_bytes;  /* HBM for in-flight ct/e buffers (admission control) */
+} cmd_server_config;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, launches;
+    double   mean_batch;
+    double   p50_us, p99_us, max_us;   /* submit -> result available */
+    double   busy_s;                   /* time with at least one launch in flight */
+} cmd_server_stats;
+
+int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd_server **out);
+/* Copies the ciphertext (ct_bytes of the key's set); *req receives the request id. */
+int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *ct, uint64_t *req);
+/* CMD_OK and the error vector once done, CMD_ERR_PENDING before; frees the request. */
+int cmd_server_poll(cmd_server *s, uint64_t req, uint8_t *e_out);
+/* Blocks until done or timeout_us elapses (< 0: no limit). */
+int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *e_out, double timeout_us);
+int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out);
+int cmd_server_reset_stats(cmd_server *s);
+/* Stops admission, drains in-flight work, frees everything. */
+int cmd_server_destroy(cmd_server *s);
+
+/* ---- measurement helpers -------------------------------------------------------------- */
+/* Average duration in microseconds of the dominant kernel of the last
+ * cmd_batch_run/cmd_decrypt_device calls made with timing on (HIP events on the launch
+ * stream), and resets the accumulator. */
+int cmd_set_kernel_timing(cmd_ctx *ctx, int32_t on);
+int cmd_kernel_timing(cmd_ctx *ctx, double *avg_us, uint64_t *launches);
+
+/* Library build identification (kernel variants compiled in). */
+const char *cmd_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CM_DECRYPT_H */

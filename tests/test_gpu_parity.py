@@ -1,0 +1,130 @@
+"""GPU parity: every kernel path against the oracle (C restatement of the reference
+decoder), bit for bit, on seeded inputs.  Runs through the C ABI of libcmdecrypt.so."""
+import numpy as np
+import pytest
+
+import cm_decrypt__amd as cm
+import _oracle as orc
+
+pytestmark = pytest.mark.gpu
+PIDS = list(cm.PARAM_IDS)
+
+
+@pytest.fixture(scope="module")
+def keys(ctx):
+    out = {}
+    for pid in PIDS:
+        sk = cm.gen_keypair(pid, 7000 + pid % 1000)
+        out[pid] = (sk, ctx.load_key(pid, sk))
+    return out
+
+
+@pytest.fixture(scope="module")
+def shipped(ctx):
+    out = {}
+    for pid in PIDS:
+        sk = orc.shipped_sk(pid)
+        out[pid] = (sk, ctx.load_key(pid, sk))
+    return out
+
+
+def rand_ct(pid, count, seed):
+    p = cm.params(pid)
+    return np.random.default_rng(seed).integers(0, 256, (count, p.ct_bytes), dtype=np.uint8)
+
+
+def inputs(pid, sk, n_valid, n_rand, seed):
+    ct, e = cm.gen_encrypt(pid, sk, seed, n_valid)
+    ct = np.concatenate([ct, rand_ct(pid, n_rand, seed + 1)])
+    return ct, e
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_stage_synd(ctx, keys, pid):
+    sk, kid = keys[pid]
+    ct, _ = inputs(pid, sk, 6, 6, 11)
+    g, L, T = orc.key_expand(pid, sk)
+    got = ctx.synd(kid, ct)
+    want = np.stack([orc.synd(pid, T, c) for c in ct])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_stage_bm(ctx, keys, pid):
+    sk, kid = keys[pid]
+    ct, _ = inputs(pid, sk, 6, 6, 12)
+    g, L, T = orc.key_expand(pid, sk)
+    s = np.stack([orc.synd(pid, T, c) for c in ct])
+    s_rand = np.random.default_rng(5).integers(0, 1 << cm.params(pid).m, (4, s.shape[1]), dtype=np.uint16)
+    s = np.concatenate([s, s_rand, np.zeros((1, s.shape[1]), np.uint16)])
+    got = ctx.bm(pid, s)
+    want = np.stack([orc.bm(pid, x) for x in s])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_stage_root(ctx, keys, pid):
+    sk, kid = keys[pid]
+    ct, _ = inputs(pid, sk, 4, 4, 13)
+    g, L, T = orc.key_expand(pid, sk)
+    loc = np.stack([orc.bm(pid, orc.synd(pid, T, c)) for c in ct])
+    got = ctx.root(kid, loc)
+    want = np.stack([orc.error_vector(pid, orc.root(pid, f, L)) for f in loc])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("mode", [cm.MODE_PLAIN, cm.MODE_FUSED])
+@pytest.mark.parametrize("pid", PIDS)
+def test_decrypt_one_key(ctx, keys, pid, mode):
+    sk, kid = keys[pid]
+    ct, e = inputs(pid, sk, 37, 8, 21)          # 45 items: one full + one partial tile
+    got = ctx.decrypt([kid] * len(ct), ct, mode=mode)
+    want = orc.decrypt(pid, sk, ct)
+    assert np.array_equal(got[:, :want.shape[1]], want)
+    assert np.array_equal(got[:37, :e.shape[1]], e)   # valid ciphertexts recover the error exactly
+
+
+@pytest.mark.parametrize("mode", [cm.MODE_PLAIN, cm.MODE_FUSED])
+def test_decrypt_shipped_keys(ctx, shipped, mode):
+    for pid in PIDS:
+        sk, kid = shipped[pid]
+        ct, e = cm.gen_encrypt(pid, sk, 31, 5)
+        got = ctx.decrypt([kid] * 5, ct, mode=mode)
+        assert np.array_equal(got, e), pid
+
+
+@pytest.mark.parametrize("mode", [cm.MODE_PLAIN, cm.MODE_FUSED])
+def test_decrypt_mixed_sets_and_keys(ctx, keys, shipped, mode):
+    rng = np.random.default_rng(3)
+    rows, key_ids, want = [], [], []
+    for pid in PIDS:
+        for (sk, kid), n in ((keys[pid], 19), (shipped[pid], 7)):
+            ct, e = inputs(pid, sk, n, 2, 40 + n)
+            w = orc.decrypt(pid, sk, ct)
+            for c, x in zip(ct, w):
+                r = np.zeros(208, np.uint8)
+                r[:c.size] = c
+                rows.append(r)
+                key_ids.append(kid)
+                wx = np.zeros(1024, np.uint8)
+                wx[:x.size] = x
+                want.append(wx)
+    order = rng.permutation(len(rows))
+    ct = np.stack(rows)[order]
+    kids = np.array(key_ids, np.int32)[order]
+    want = np.stack(want)[order]
+    got = ctx.decrypt(kids, ct, mode=mode, ct_stride=208, e_stride=1024)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_weight_below_t(ctx, keys, pid):
+    """w < t leaves a factor x^(t-w) in the reversed locator: a support element equal to 0
+    is reported as an error too (the reference's behaviour, reproduced bit for bit)."""
+    sk, kid = keys[pid]
+    t = cm.params(pid).t
+    ct, e = cm.gen_encrypt(pid, sk, 77, 6, weight=t - 5)
+    want = orc.decrypt(pid, sk, ct)
+    for mode in (cm.MODE_PLAIN, cm.MODE_FUSED):
+        got = ctx.decrypt([kid] * len(ct), ct, mode=mode)
+        assert np.array_equal(got, want)
