@@ -76,24 +76,80 @@ def time_batch(torch, batch, d_ct, d_e, steps, warmup):
     return wall, kern_ms
 
 
-def serve_leg(cm, ctx, rates, n_keys, duration_s, seed, threads):
-    """Poisson arrivals of the 5-set mix over n_keys resident keys; p50/p99 per rate."""
-    rng = np.random.default_rng(seed)
+def key_pool(cm, ctx, n_keys, seed, threads):
+    """n_keys resident keys split over the 5-set mix: {pid: [(key_id, sk), ...]}."""
     per_set = {pid: max(1, int(round(n_keys * w / sum(x[1] for x in MIX)))) for pid, w in MIX}
-    pool = []  # (key_id, pid, ct, e)
-    t0 = time.perf_counter()
+    pool = {}
     for pid, nk in per_set.items():
         sks = cm.gen_keys(pid, seed + pid % 1000, nk, threads=threads)
+        pool[pid] = [(ctx.load_key(pid, sks[k]), sks[k]) for k in range(nk)]
+    return pool
 
+
+def mixed_leg(cm, torch, ctx, pool, n_items, keys_per_set, steps, seed, threads):
+    """One launch over a shuffled mix of all five sets (30/20/20/15/15 %), keys_per_set keys
+    each, exact-weight-t errors; dec/s and bit-exactness against the generated errors."""
+    rng = np.random.default_rng(seed)
+    counts = {pid: int(round(n_items * w)) for pid, w in MIX}
+    counts[MIX[0][0]] += n_items - sum(counts.values())
+    jobs = []
+    for pid, cnt in counts.items():
+        keys = pool[pid][:keys_per_set]
+        per = [cnt // len(keys) + (1 if k < cnt % len(keys) else 0) for k in range(len(keys))]
+        jobs += [(pid, kid, sk, c, 1000 + i) for i, ((kid, sk), c) in enumerate(zip(keys, per)) if c]
+
+    def enc(job):
+        pid, kid, sk, c, sd = job
+        ct, e = cm.gen_encrypt(pid, sk, seed * 131 + sd, c)
+        return pid, kid, ct, e
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(enc, jobs))
+    ct = np.zeros((n_items, 208), np.uint8)
+    e_true = np.zeros((n_items, 1024), np.uint8)
+    kids = np.zeros(n_items, np.int32)
+    order = rng.permutation(n_items)
+    pos = 0
+    for pid, kid, c, e in parts:
+        rows = order[pos:pos + len(c)]
+        ct[rows, :c.shape[1]] = c
+        e_true[rows, :e.shape[1]] = e
+        kids[rows] = kid
+        pos += len(c)
+    d_ct = torch.from_numpy(ct).to("cuda")
+    d_e = torch.zeros((n_items, 1024), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    b = cm.Batch(ctx, kids, 208, 1024, mode=cm.MODE_FUSED, use_graph=True)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        wall, kern_ms = time_batch(torch, b, d_ct, d_e, steps, 1)
+    got = d_e.cpu().numpy()
+    exact = True
+    for i in range(0, n_items, 4096):
+        sl = slice(i, i + 4096)
+        exact &= bool(np.array_equal(got[sl], e_true[sl]))
+    b.close()
+    return {"items": n_items, "keys": keys_per_set * len(MIX), "value": round(n_items * steps / wall, 1),
+            "ms_per_step": round(wall / steps * 1e3, 3), "kernel_ms": round(kern_ms, 3), "bit_exact": exact,
+            "mix": {str(pid): counts[pid] for pid, _ in MIX}}
+
+
+def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads):
+    """Poisson arrivals of the 5-set mix over the resident key pool; p50/p99 per rate."""
+    rng = np.random.default_rng(seed)
+    per_set = {pid: len(keys) for pid, keys in pool.items()}
+    templates = []  # (key_id, pid, ct, e)
+    t0 = time.perf_counter()
+    for pid, keys in pool.items():
         def enc(k):
-            return cm.gen_encrypt(pid, sks[k], seed + 31 * k, 4)
+            return cm.gen_encrypt(pid, keys[k][1], seed + 31 * k, 4)
 
         with cf.ThreadPoolExecutor(threads) as ex:
-            encs = list(ex.map(enc, range(nk)))
-        for k in range(nk):
-            kid = ctx.load_key(pid, sks[k])
+            encs = list(ex.map(enc, range(len(keys))))
+        for k, (kid, _) in enumerate(keys):
             for c, x in zip(*encs[k]):
-                pool.append((kid, pid, c, x))
+                templates.append((kid, pid, c, x))
+    pool = templates
     log(f"serving: {len(pool)} request templates over {sum(per_set.values())} resident keys "
         f"({time.perf_counter() - t0:.1f}s setup)")
     weights = np.array([dict(MIX)[p[1]] / per_set[p[1]] for p in pool])
@@ -173,6 +229,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serve-keys", type=int, default=1024)
     ap.add_argument("--serve-rates", default="1e4,1e5,1e6")
@@ -180,6 +237,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--schedules", default=os.path.join(ROOT, "cm_decrypt__amd", "schedules.json"))
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -205,12 +264,19 @@ def main():
     d_e = torch.zeros((N_CT, p.e_bytes), dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
 
+    if args.copy_probe_mb:
+        src = torch.empty(args.copy_probe_mb << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mb} MiB device-to-device")
+        del src, dst
     stream = torch.cuda.Stream()  # default priority; graphs capture on this stream
     tuned = None
     if not args.no_autotune:
-        best, best_us = ctx.autotune(SET, key_ids[:16384], reps=2)
-        tuned = {"schedule": best, "us_16384": round(best_us, 1)}
-        log(f"autotuned schedule {best} ({best_us:.0f} us for 16384 items)")
+        best, best_us = ctx.autotune(SET, key_ids, reps=2)
+        tuned = {"schedule": best, "best_us": round(best_us, 1), "items": int(N_CT)}
+        log(f"autotuned schedule {best} ({best_us:.0f} us for {N_CT} items)")
     sched = ctx.schedule(SET)
 
     fused = cm.Batch(ctx, key_ids, p.ct_bytes, p.e_bytes, mode=cm.MODE_FUSED, use_graph=True)
@@ -235,10 +301,18 @@ def main():
         log(f"plain: {pval:,.0f} dec/s ({plain['ms_per_step']:.1f} ms/step), fused/plain {value / pval:.2f}x")
         pb.close()
 
-    serve = None
-    if not args.no_serve:
-        rates = [float(x) for x in args.serve_rates.split(",") if x]
-        serve = serve_leg(cm, ctx, rates, args.serve_keys, args.serve_seconds, 17, threads)
+    serve = mixed = None
+    if not (args.no_serve and args.no_mixed):
+        t0 = time.perf_counter()
+        pool = key_pool(cm, ctx, args.serve_keys, 17, threads)
+        log(f"key pool: {sum(len(v) for v in pool.values())} resident keys ({time.perf_counter() - t0:.1f}s)")
+        if not args.no_mixed:
+            mixed = mixed_leg(cm, torch, ctx, pool, 100000, 64, max(2, args.steps // 2), 23, threads)
+            log(f"mixed 5-set batch: {mixed['value']:,.0f} dec/s ({mixed['ms_per_step']:.2f} ms/step) "
+                f"exact={mixed['bit_exact']}")
+        if not args.no_serve:
+            rates = [float(x) for x in args.serve_rates.split(",") if x]
+            serve = serve_leg(cm, ctx, pool, rates, args.serve_seconds, 17, threads)
 
     cpu = None if args.no_cpu else cpu_baseline(cm, sks, ct, key_of, args.cpu_seconds)
 
@@ -276,6 +350,7 @@ def main():
                              "256 CU x 64 lanes x 2.4 GHz"},
         "roofline_hbm": {"bound": "hbm", "achieved": round(hbm_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "traffic": None},
+        "mixed_batch": mixed,
         "serve": serve,
         "cpu_baseline": cpu,
     }

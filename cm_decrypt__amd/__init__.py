@@ -57,6 +57,12 @@ class CmdError(RuntimeError):
 
 
 def _load():
+    # torch (when installed) bundles its own HIP runtime under the same soname; loading it
+    # first makes this library bind to that one runtime instead of a second copy.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (or __graft_entry__.build())")
@@ -76,7 +82,8 @@ class ParamInfo(_c.Structure):
 
 
 class Schedule(_c.Structure):
-    _fields_ = [(n, _c.c_int32) for n in ("tile_cts", "waves_per_group", "waves_per_ct", "groups_per_cu")]
+    _fields_ = [(n, _c.c_int32) for n in ("tile_cts", "waves_per_group", "waves_per_ct", "groups_per_cu",
+                                          "root_alg")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

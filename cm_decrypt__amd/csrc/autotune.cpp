@@ -17,14 +17,15 @@ using namespace cmd;
 static std::vector<FusedSchedule> candidates()
 {
     std::vector<FusedSchedule> out;
-    for (int tile : {16, 32})
-        for (int wpg : {1, 2, 4})
-            for (int gpc : {1, 2, 4, 8}) {
-                FusedSchedule s{tile, wpg, 1, gpc};
-                // LDS per workgroup is wpg * 16.4 KB; keep resident groups within 160 KB per CU
-                if (wpg * gpc > 9) continue;
-                if (fused_supported(s)) out.push_back(s);
-            }
+    for (int alg : {1, 0})
+        for (int tile : {16, 32})
+            for (int wpg : {1, 2, 4})
+                for (int gpc : {1, 2, 4, 8}) {
+                    FusedSchedule s{tile, wpg, 1, gpc, alg};
+                    // LDS per workgroup is wpg * 16.4 KB; keep resident groups within 160 KB per CU
+                    if (wpg * gpc > 9) continue;
+                    if (fused_supported(s)) out.push_back(s);
+                }
     return out;
 }
 
@@ -101,6 +102,7 @@ extern "C" int cmd_autotune(cmd_ctx *ctx, int32_t param_id, size_t count, const 
         best->waves_per_group = winner.waves_per_group;
         best->waves_per_ct = winner.waves_per_ct;
         best->groups_per_cu = winner.groups_per_cu;
+        best->root_alg = winner.root_alg;
     }
     if (best_us) *best_us = winner_us;
     return CMD_OK;
@@ -116,8 +118,8 @@ extern "C" int cmd_schedule_save(cmd_ctx *ctx, const char *path)
     for (int i = 0; i < kNumParams; i++) {
         const FusedSchedule &s = ctx->sched[i];
         std::fprintf(f, "  \"%d\": {\"tile_cts\": %d, \"waves_per_group\": %d, \"waves_per_ct\": %d, "
-                        "\"groups_per_cu\": %d}%s\n",
-                     kParams[i].id, s.tile_cts, s.waves_per_group, s.waves_per_ct, s.groups_per_cu,
+                        "\"groups_per_cu\": %d, \"root_alg\": %d}%s\n",
+                     kParams[i].id, s.tile_cts, s.waves_per_group, s.waves_per_ct, s.groups_per_cu, s.root_alg,
                      i + 1 < kNumParams ? "," : "");
     }
     std::fprintf(f, "}\n");
@@ -134,14 +136,14 @@ extern "C" int cmd_schedule_load(cmd_ctx *ctx, const char *path)
     int loaded = 0;
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     while (std::fgets(line, sizeof line, f)) {
-        int id, tile, wpg, wpc, gpc;
+        int id, tile, wpg, wpc, gpc, alg;
         const char *q = std::strchr(line, '"');
         if (!q) continue;
         if (std::sscanf(q, "\"%d\": {\"tile_cts\": %d, \"waves_per_group\": %d, \"waves_per_ct\": %d, "
-                           "\"groups_per_cu\": %d}",
-                        &id, &tile, &wpg, &wpc, &gpc) == 5) {
+                           "\"groups_per_cu\": %d, \"root_alg\": %d}",
+                        &id, &tile, &wpg, &wpc, &gpc, &alg) == 6) {
             int pidx = param_index(id);
-            FusedSchedule s{tile, wpg, wpc, gpc};
+            FusedSchedule s{tile, wpg, wpc, gpc, alg};
             if (pidx >= 0 && fused_supported(s) && gpc >= 1 && gpc <= 16) {
                 ctx->sched[pidx] = s;
                 loaded++;

@@ -71,6 +71,7 @@ FusedSchedule default_schedule(int pidx)
     s.waves_per_group = 4;
     s.waves_per_ct = 1;
     s.groups_per_cu = 2;
+    s.root_alg = 1;
     return s;
 }
 
@@ -80,8 +81,9 @@ static size_t key_device_bytes(int pidx)
     size_t table = (size_t)p.sb * 2 * p.t * 2;
     size_t L = (size_t)p.n * 2;
     size_t lb = (size_t)p.m * lbits_stride(pidx) * 4;
+    size_t bn = (size_t)(2 * p.m - 1) * ((size_t)1 << (p.m - 5)) * 4;
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-    return up(table) + up(L) + up(lb);
+    return up(table) + up(L) + up(lb) + up(bn);
 }
 
 static int upload_key_slot(cmd_ctx *ctx, int32_t id)
@@ -257,6 +259,7 @@ int batch_enqueue(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
     a.ct_stride = b->ct_stride;
     a.e_stride = b->e_stride;
     a.tile_counter = b->d_counter;
+    a.fft = ctx->d_fft;
     const FusedSchedule s = launch_schedule(b->sched, b->tiles, b->launch_tile);
     CMD_HIP(launch_zero_u32(b->d_counter, 1, st));
     CMD_HIP(launch_fused(a, s, ctx->n_cu, st));
@@ -404,6 +407,14 @@ extern "C" int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx
     CMD_HIP(hipEventCreate(&ctx->ev1));
     for (int i = 0; i < kNumParams; i++) ctx->sched[i] = default_schedule(i);
     CMD_HIP(hipMalloc(&ctx->d_keys, kMaxKeys * sizeof(KeyDev)));
+    {
+        std::vector<FftTab> tabs(kFftVariants);
+        build_fft_tab(12, 7, &tabs[0]);
+        build_fft_tab(13, 7, &tabs[1]);
+        build_fft_tab(13, 8, &tabs[2]);
+        CMD_HIP(hipMalloc(&ctx->d_fft, kFftVariants * sizeof(FftTab)));
+        CMD_HIP(hipMemcpy(ctx->d_fft, tabs.data(), kFftVariants * sizeof(FftTab), hipMemcpyHostToDevice));
+    }
     *out = ctx.release();
     return CMD_OK;
 }
@@ -416,6 +427,7 @@ extern "C" int cmd_ctx_destroy(cmd_ctx *ctx)
     for (auto &k : ctx->keys)
         if (k.mem) (void)hipFree(k.mem);
     if (ctx->d_keys) (void)hipFree(ctx->d_keys);
+    if (ctx->d_fft) (void)hipFree(ctx->d_fft);
     for (DevBuf *b : {&ctx->ct, &ctx->e, &ctx->s, &ctx->loc, &ctx->idx, &ctx->plan}) dev_free(ctx, *b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -462,12 +474,15 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
     const ParamDesc &p = kParams[pidx];
     char *base = (char *)mem;
     size_t off_L = up((size_t)p.sb * 2 * p.t * 2), off_lb = off_L + up((size_t)p.n * 2);
+    size_t off_bn = off_lb + up(K.lbits.size() * 4);
     hipError_t e1 = hipMemcpy(base, K.table.data(), K.table.size() * 2, hipMemcpyHostToDevice);
     hipError_t e2 = hipMemcpy(base + off_L, K.L.data(), K.L.size() * 2, hipMemcpyHostToDevice);
     hipError_t e3 = hipMemcpy(base + off_lb, K.lbits.data(), K.lbits.size() * 4, hipMemcpyHostToDevice);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    hipError_t e4 = hipMemcpy(base + off_bn, K.benes.data(), K.benes.size() * 4, hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
         (void)hipFree(mem);
-        return hip_fail(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "key upload");
+        hipError_t bad = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : (e3 != hipSuccess ? e3 : e4));
+        return hip_fail(bad, "key upload");
     }
     KeySlot slot;
     slot.used = true;
@@ -477,6 +492,7 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
     slot.dev.table = (const uint16_t *)base;
     slot.dev.L = (const uint16_t *)(base + off_L);
     slot.dev.lbits = (const uint32_t *)(base + off_lb);
+    slot.dev.benes = (const uint32_t *)(base + off_bn);
     slot.dev.pidx = pidx;
     slot.dev.lstride = lbits_stride(pidx);
     int32_t id = -1;
@@ -642,6 +658,8 @@ extern "C" int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32
     if (!rc) rc = dev_reserve(ctx, ctx->e, count * e_stride);
     if (rc) return rc;
     CMD_HIP(hipMemcpyAsync(ctx->ct.ptr, ct, count * ct_stride, hipMemcpyHostToDevice, ctx->stream));
+    // rows come back whole: bytes past an item's n/8 read as zero rather than stale data
+    CMD_HIP(hipMemsetAsync(ctx->e.ptr, 0, count * e_stride, ctx->stream));
     rc = cmd_decrypt_device(ctx, mode, count, key_ids, ctx->ct.ptr, ct_stride, ctx->e.ptr, e_stride, ctx->stream);
     if (rc) return rc;
     CMD_HIP(hipMemcpyAsync(e_out, ctx->e.ptr, count * e_stride, hipMemcpyDeviceToHost, ctx->stream));
@@ -736,6 +754,7 @@ extern "C" int cmd_schedule_get(cmd_ctx *ctx, int32_t param_id, cmd_schedule *ou
     out->waves_per_group = s.waves_per_group;
     out->waves_per_ct = s.waves_per_ct;
     out->groups_per_cu = s.groups_per_cu;
+    out->root_alg = s.root_alg;
     return CMD_OK;
 }
 
@@ -744,7 +763,7 @@ extern "C" int cmd_schedule_set(cmd_ctx *ctx, int32_t param_id, const cmd_schedu
     int pidx = param_index(param_id);
     if (!ctx || !in) return CMD_ERR_ARG;
     if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
-    FusedSchedule s{in->tile_cts, in->waves_per_group, in->waves_per_ct, in->groups_per_cu};
+    FusedSchedule s{in->tile_cts, in->waves_per_group, in->waves_per_ct, in->groups_per_cu, in->root_alg};
     if (!fused_supported(s) || s.groups_per_cu < 1 || s.groups_per_cu > 16)
         return set_error(CMD_ERR_ARG, "schedule (%d, %d, %d, %d) not available", s.tile_cts, s.waves_per_group,
                          s.waves_per_ct, s.groups_per_cu);

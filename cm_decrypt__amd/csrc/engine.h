@@ -63,6 +63,7 @@ struct cmd_ctx {
     uint32_t *d_counter = nullptr;
 
     cmd::FusedSchedule sched[cmd::kNumParams];
+    cmd::FftTab *d_fft = nullptr;   // kFftVariants additive-FFT constant tables
 
     bool timing = false;
     double timing_us = 0.0;

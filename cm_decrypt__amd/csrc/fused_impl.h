@@ -1,0 +1,744 @@
+// Fused persistent decryption kernel (device code shared by the kernel variants):
+// syndrome -> Berlekamp-Massey -> root search in one launch, for any mix of parameter sets
+// and keys.
+//
+// Work unit: a tile of up to 32 ciphertexts of one parameter set (the host may plan smaller
+// tiles), handled by one wavefront from start to finish.  Wavefronts pull tiles from a global counter until
+// the list is exhausted (every wave exits when the counter passes n_tiles).
+//
+//  1. syndrome (CPU/param/param6960119/src/util.c:102-114): lane l owns table columns
+//     2c, 2c+1 for c = l + 64r; each table row is streamed once per 16 ciphertexts and
+//     masked with wave-uniform ciphertext bits.  The 2t syndromes are transposed into LDS
+//     bit planes: word [j][k] holds bit k of s_j for every ciphertext of the tile (bit x =
+//     tile slot x).
+//  2. Berlekamp-Massey (CPU/src/common/bm.c:12-68), bit-sliced across the tile: lane l holds
+//     planes of coefficients l + 64b of C and B; discrepancies reduce over lanes with DPP,
+//     the per-ciphertext discrepancy, masks mne/mle, L and b live in scalar registers.  Inversion-free
+//     update C' = b*C + d*B keeps every iterate a nonzero multiple of the reference's C, so
+//     the zeros of the locator (all that the error vector depends on) are the reference's.
+//  3. root search (root_13.c:233-240, decrypt.c:43-53): for each ciphertext, lane l evaluates
+//     the reversed locator by Horner at 32 support points at once (bit-sliced L planes of the
+//     key); a position is an error iff all m planes are zero there, which yields the packed
+//     error-vector word directly.
+// No step indexes memory or branches on secret data; the control flow depends only on
+// public sizes and the iteration counter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "dev_gf.h"
+#include "kernels.h"
+#include "params.h"
+
+namespace cmd {
+
+constexpr int kRowWords = 16;                          // LDS words per syndrome plane row (m used)
+constexpr int kWaveLds = (2 * kMaxT + 1) * kRowWords;  // words of LDS per wavefront
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp0(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+
+// XOR over the 64 lanes, returned as a wave-uniform value.
+__device__ __forceinline__ uint32_t wave_xor_all(uint32_t v)
+{
+    v ^= dpp0<0xB1>(v);   // quad_perm [1,0,3,2]
+    v ^= dpp0<0x4E>(v);   // quad_perm [2,3,0,1]
+    v ^= dpp0<0x124>(v);  // row_ror:4
+    v ^= dpp0<0x128>(v);  // row_ror:8
+    return (uint32_t)(__builtin_amdgcn_readlane((int)v, 0) ^ __builtin_amdgcn_readlane((int)v, 16) ^
+                      __builtin_amdgcn_readlane((int)v, 32) ^ __builtin_amdgcn_readlane((int)v, 48));
+}
+
+// Lane l receives v from lane l-1; lane 0 receives `carry`.
+__device__ __forceinline__ uint32_t shift_up1(uint32_t v, uint32_t carry)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)carry, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Bit-sliced "L <= K" over 9-bit values L (planes Lp), K a public constant.
+__device__ __forceinline__ uint32_t bs_le_const(const uint32_t Lp[9], uint32_t K)
+{
+    uint32_t lt = 0, eq = ~0u;
+#pragma unroll
+    for (int q = 8; q >= 0; q--) {
+        if ((K >> q) & 1u) {
+            lt |= eq & ~Lp[q];
+            eq &= Lp[q];
+        } else {
+            eq &= ~Lp[q];
+        }
+    }
+    return lt | eq;
+}
+
+// Bit-sliced V - L for a public constant V (9 bits, modulo 512).
+__device__ __forceinline__ void bs_const_sub(uint32_t V, const uint32_t Lp[9], uint32_t out[9])
+{
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const uint32_t l = Lp[q];
+        if ((V >> q) & 1u) {
+            out[q] = ~(l ^ borrow);
+            borrow = l & borrow;
+        } else {
+            out[q] = l ^ borrow;
+            borrow = l | borrow;
+        }
+    }
+}
+
+
+// Value of lane (l ^ OFF), OFF a power of two below 64.
+template <int OFF>
+__device__ __forceinline__ uint32_t xor_lanes(uint32_t v)
+{
+    if constexpr (OFF == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    } else if constexpr (OFF == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    } else if constexpr (OFF == 4) {
+        const int y = __builtin_amdgcn_mov_dpp((int)v, 0x1B, 0xF, 0xF, true);     // l ^ 3
+        return (uint32_t)__builtin_amdgcn_mov_dpp(y, 0x141, 0xF, 0xF, true);     // row_half_mirror: l ^ 7
+    } else if constexpr (OFF == 8) {
+        const int y = __builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, true);    // row_mirror: l ^ 15
+        return (uint32_t)__builtin_amdgcn_mov_dpp(y, 0x141, 0xF, 0xF, true);     // then l ^ 7
+    } else if constexpr (OFF == 16) {
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (16 << 10) | 0x1F);  // bitmask mode, xor 16
+    } else {
+        static_assert(OFF == 32, "lane offset");
+        const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, (int)v);
+    }
+}
+
+// Value of lane (l ^ OFF) for any OFF < 64 (composed from the power-of-two exchanges).
+template <int OFF>
+__device__ __forceinline__ uint32_t xor_lanes_any(uint32_t v)
+{
+    if constexpr (OFF == 0) {
+        return v;
+    } else if constexpr (OFF == 3) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x1B, 0xF, 0xF, true);  // quad_perm [3,2,1,0]
+    } else {
+        constexpr int low = OFF & -OFF;
+        return xor_lanes<low>(xor_lanes_any<OFF - low>(v));
+    }
+}
+
+// a <- a * c for a per-lane constant c (bit-sliced over 32 elements sharing c)
+template <int M>
+__device__ __forceinline__ void bs_mul_scalar(uint32_t a[M], uint32_t c)
+{
+    uint32_t cm[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) cm[k] = bitmask(c, k);
+    bs_mul<M>(a, cm, a);
+}
+
+__device__ __forceinline__ uint32_t bitrev_bits(uint32_t v, int bits)
+{
+    return __builtin_bitreverse32(v) >> (32 - bits);
+}
+
+template <int IDX, int ALG>
+struct FusedTileKernel {
+    static constexpr int TILE = 32;  // bit-sliced width (ciphertext slots per wavefront tile)
+    using Pm = P<IDX>;
+    static constexpr int M = Pm::m, T = Pm::t, N = Pm::n, G = Pm::groups;
+    static constexpr int NBL = (T + 1 + 63) / 64;        // lane blocks of 64 coefficients
+    static constexpr int PAIRS = T;                      // table dwords per row (2t uint16)
+    static constexpr int R = (PAIRS + 63) / 64;
+    static constexpr int WORDS = (Pm::synd_bytes + 3) / 4;
+    static constexpr int CH = TILE < 16 ? TILE : 16;     // ciphertexts per syndrome chunk
+    static constexpr int CST = (T + 2) & ~1;             // LDS uint16 stride of a coefficient row
+    static_assert(TILE * CST * 2 <= kWaveLds * 4, "coefficient rows must fit the wave's LDS");
+
+    // ---- stage 1: syndromes into LDS bit planes ---------------------------------------------
+    static __device__ __forceinline__ void syndrome(const FusedArgs &a, const FusedTile &tl, bool one_key, uint32_t *lds,
+                                    int lane)
+    {
+        // row 0: the zero row read for s_{negative}
+        if (lane < kRowWords) lds[lane] = 0;
+        const int chunks = (tl.count + CH - 1) / CH;
+        for (int c = 0; c < chunks; c++) {
+            const int x0 = c * CH;
+            const uint8_t *my_ct = nullptr;
+            if (lane < CH && x0 + lane < tl.count)
+                my_ct = a.ct + (size_t)a.items[tl.first + x0 + lane] * a.ct_stride;
+
+            uint32_t acc[CH][R];
+#pragma unroll
+            for (int x = 0; x < CH; x++)
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[x][r] = 0;
+
+            if (one_key) {
+                const KeyDev *kd = &a.keys[a.item_key[tl.first + x0]];
+                const uint32_t *rows = reinterpret_cast<const uint32_t *>(kd->table);
+                for (int w = 0; w < WORDS; w++) {
+                    uint32_t mine = 0;
+                    if (my_ct) {
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            if (4 * w + b < Pm::synd_bytes) mine |= (uint32_t)my_ct[4 * w + b] << (8 * b);
+                    }
+                    uint32_t cw[CH];
+#pragma unroll
+                    for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
+                    const int nbits = (w * 32 + 32 <= Pm::sb) ? 32 : (Pm::sb - w * 32);
+#pragma unroll 8
+                    for (int bit = 0; bit < nbits; bit++) {
+                        const uint32_t *row = rows + (size_t)(w * 32 + bit) * PAIRS;
+                        uint32_t v[R];
+#pragma unroll
+                        for (int r = 0; r < R; r++) v[r] = (lane + 64 * r < PAIRS) ? row[lane + 64 * r] : 0u;
+#pragma unroll
+                        for (int x = 0; x < CH; x++) {
+                            const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x], bit, 1);
+#pragma unroll
+                            for (int r = 0; r < R; r++) acc[x][r] ^= v[r] & msk;
+                        }
+                    }
+                }
+            } else {
+                // one key per ciphertext: each row load feeds one accumulator, so issue a whole
+                // ciphertext word's 32 rows at once to keep many loads in flight
+#pragma unroll
+                for (int x = 0; x < CH; x++) {
+                    if (x0 + x >= tl.count) continue;
+                    const KeyDev *kd = &a.keys[a.item_key[tl.first + x0 + x]];
+                    const uint32_t *rows = reinterpret_cast<const uint32_t *>(kd->table);
+                    const uint8_t *ctp = a.ct + (size_t)a.items[tl.first + x0 + x] * a.ct_stride;
+                    for (int w = 0; w < WORDS; w++) {
+                        uint32_t cwv = 0;
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            if (4 * w + b < Pm::synd_bytes) cwv |= (uint32_t)ctp[4 * w + b] << (8 * b);
+                        const uint32_t cw = uni(cwv);
+                        if (w * 32 + 32 <= Pm::sb) {
+                            uint32_t v[32][R];
+#pragma unroll
+                            for (int bit = 0; bit < 32; bit++)
+#pragma unroll
+                                for (int r = 0; r < R; r++)
+                                    v[bit][r] = (lane + 64 * r < PAIRS) ? rows[(size_t)(w * 32 + bit) * PAIRS + lane + 64 * r] : 0u;
+#pragma unroll
+                            for (int bit = 0; bit < 32; bit++) {
+                                const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw, bit, 1);
+#pragma unroll
+                                for (int r = 0; r < R; r++) acc[x][r] ^= v[bit][r] & msk;
+                            }
+                        } else {
+                            for (int bit = 0; bit < Pm::sb - w * 32; bit++) {
+                                const uint32_t *row = rows + (size_t)(w * 32 + bit) * PAIRS;
+                                const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw, bit, 1);
+#pragma unroll
+                                for (int r = 0; r < R; r++)
+                                    acc[x][r] ^= ((lane + 64 * r < PAIRS) ? row[lane + 64 * r] : 0u) & msk;
+                            }
+                        }
+                    }
+                }
+            }
+
+            // transpose: plane (j, k) gets bit k of s_j of chunk ciphertext x at bit x0 + x
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int cp = lane + 64 * r;
+                if (cp >= PAIRS) continue;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int j = 2 * cp + h;
+#pragma unroll
+                    for (int k = 0; k < M; k++) {
+                        uint32_t plane = 0;
+#pragma unroll
+                        for (int x = 0; x < CH; x++) plane |= ((acc[x][r] >> (16 * h + k)) & 1u) << x;
+                        uint32_t *dst = &lds[(j + 1) * kRowWords + k];
+                        if constexpr (TILE == 32) {
+                            reinterpret_cast<uint16_t *>(dst)[c] = (uint16_t)plane;
+                        } else {
+                            *dst = plane;
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- stage 2: bit-sliced Berlekamp-Massey; leaves coefficient rows in LDS ----------------
+    // Lane l holds coefficient i = l + 64b of C and B (b < NBL).  At iteration n only
+    // coefficients i <= n+1 can be nonzero, so block b joins once 64b <= n+1 (a public bound).
+    static __device__ __forceinline__ void berlekamp_massey(uint32_t *lds, int lane, uint32_t (&C)[NBL][M])
+    {
+        uint32_t B[NBL][M];
+#pragma unroll
+        for (int b = 0; b < NBL; b++)
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                C[b][k] = (b == 0 && k == 0 && lane == 0) ? ~0u : 0u;  // C = 1
+                B[b][k] = (b == 0 && k == 0 && lane == 1) ? ~0u : 0u;  // B = x
+            }
+        uint32_t bb[M], Lp[9];
+#pragma unroll
+        for (int k = 0; k < M; k++) bb[k] = k == 0 ? ~0u : 0u;
+#pragma unroll
+        for (int q = 0; q < 9; q++) Lp[q] = 0;
+
+        // coefficients above T (last block) are dropped like the reference's fixed t+1 array
+        const uint32_t keep_last = (lane + 64 * (NBL - 1) <= T) ? ~0u : 0u;
+        const uint4 *rows4 = reinterpret_cast<const uint4 *>(lds);
+
+#pragma unroll 1
+        for (int n = 0; n < 2 * T; n++) {
+            // discrepancy d = sum_i C_i s_{n-i} (rows hold s_{j} at j+1, row 0 is zero)
+            uint32_t p[2 * M - 1];
+#pragma unroll
+            for (int s2 = 0; s2 < 2 * M - 1; s2++) p[s2] = 0;
+#pragma unroll
+            for (int b = 0; b < NBL; b++) {
+                if (64 * b > n + 1) continue;
+                const int i = lane + 64 * b;
+                const int row = (n - i + 1) > 0 ? (n - i + 1) : 0;
+                uint32_t sv[16];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint4 w = rows4[row * 4 + q];
+                    sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
+                }
+                bs_mac_wide<M>(C[b], sv, p);
+            }
+            uint32_t part[M], d[M];
+            bs_fold<M>(p, part);
+#pragma unroll
+            for (int k = 0; k < M; k++) d[k] = wave_xor_all(part[k]);
+            uint32_t mne = 0;
+#pragma unroll
+            for (int k = 0; k < M; k++) mne |= d[k];
+            const uint32_t mle = mne & bs_le_const(Lp, (uint32_t)n >> 1);
+
+            // C' = b*C + d*B ; B' = mle ? C : B ; then B <- x*B
+#pragma unroll
+            for (int b = NBL - 1; b >= 0; b--) {
+                if (64 * b > n + 1) continue;
+                uint32_t cn[M];
+                bs_mul_add2<M>(bb, C[b], d, B[b], cn);
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    B[b][k] = (C[b][k] & mle) | (B[b][k] & ~mle);
+                    C[b][k] = cn[k];
+                }
+            }
+            // shift the highest block first so each carry is read before its source moves
+#pragma unroll
+            for (int b = NBL - 1; b >= 0; b--) {
+                if (64 * b > n + 1) continue;
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    const uint32_t carry =
+                        b == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)B[b > 0 ? b - 1 : 0][k], 63);
+                    B[b][k] = shift_up1(B[b][k], carry);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < M; k++) B[NBL - 1][k] &= keep_last;
+            // L, b
+            uint32_t Ln[9];
+            bs_const_sub((uint32_t)n + 1, Lp, Ln);
+#pragma unroll
+            for (int q = 0; q < 9; q++) Lp[q] = (Ln[q] & mle) | (Lp[q] & ~mle);
+#pragma unroll
+            for (int k = 0; k < M; k++) bb[k] = (d[k] & mle) | (bb[k] & ~mle);
+        }
+    }
+
+    // Coefficient rows coef[x][i] (i <= T) for ciphertext slot x, from the bit-sliced C.
+    static __device__ __forceinline__ void write_coef_rows(const uint32_t (&C)[NBL][M], uint32_t *lds, int lane)
+    {
+        wave_sync();
+        uint16_t *coef = reinterpret_cast<uint16_t *>(lds);
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+            const int i = lane + 64 * b;
+            if (i > T) continue;
+#pragma unroll 2
+            for (int x = 0; x < TILE; x++) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int k = 0; k < M; k++) v |= ((C[b][k] >> x) & 1u) << k;
+                coef[x * CST + i] = (uint16_t)v;
+            }
+        }
+        wave_sync();
+    }
+
+    // ---- stage 3: root search, 32 support points per lane per pass ----------------------------
+    static __device__ __forceinline__ void roots(const FusedArgs &a, const FusedTile &tl, const uint32_t *lds, int lane,
+                                 bool e_aligned)
+    {
+        const uint16_t *coef = reinterpret_cast<const uint16_t *>(lds);
+        for (int x = 0; x < tl.count; x++) {
+            const int item = a.items[tl.first + x];
+            const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
+            const uint32_t *lb = kd->lbits;
+            const int ls = kd->lstride;
+            uint8_t *eo = a.e + (size_t)item * a.e_stride;
+            const uint16_t *cr = coef + x * CST;
+            for (int g0 = 0; g0 < G; g0 += 64) {
+                const int g = g0 + lane;
+                const bool act = g < G;
+                uint32_t av[M], r[M];
+#pragma unroll
+                for (int k = 0; k < M; k++) av[k] = act ? lb[k * ls + g] : 0u;
+                const uint32_t c0 = uni(cr[0]);
+#pragma unroll
+                for (int k = 0; k < M; k++) r[k] = bitmask(c0, k);
+                for (int j = 1; j <= T; j++) {
+                    const uint32_t cj = uni(cr[j]);
+                    bs_mul<M>(r, av, r);
+#pragma unroll
+                    for (int k = 0; k < M; k++) r[k] ^= bitmask(cj, k);
+                }
+                uint32_t nz = 0;
+#pragma unroll
+                for (int k = 0; k < M; k++) nz |= r[k];
+                uint32_t z = ~nz;
+                if (act) {
+                    const int nbytes = (N / 8 - 4 * g) < 4 ? (N / 8 - 4 * g) : 4;
+                    uint8_t *dst = eo + 4 * g;
+                    if (nbytes == 4 && e_aligned) {
+                        *reinterpret_cast<uint32_t *>(dst) = z;
+                    } else {
+                        for (int q = 0; q < nbytes; q++) dst[q] = (uint8_t)(z >> (8 * q));
+                    }
+                }
+            }
+        }
+    }
+
+
+    // ---- stage 3, additive-FFT variant --------------------------------------------------------
+    // The reversed locator f_p = C_{T-p} (the polynomial root() evaluates, root_13.c:216-240) is
+    // evaluated at every field element by the Gao-Mateer recursion (constants in FftTab):
+    //  A. coefficient phase, bit-sliced across the tile like C: per depth a twist by public
+    //     constants and the radix conversion f(x) = f0(x^2+x) + x f1(x^2+x) as XOR passes;
+    //     the 2^LOG leaves (one per recursion path) go to LDS per ciphertext;
+    //  B. per ciphertext, bit-sliced over 32 evaluation points per word: leaves broadcast, then
+    //     LOG butterfly levels w0 = u + G v, w1 = w0 + v.  Word w, bit b is evaluation index
+    //     c = 32w + b, the value at bitrev_m(c).  A zero value marks a root.
+    // The key's own Benes network (the one support_gen applies to bitrev(i), root_13.c:181-211)
+    // then moves the root indicator of index c to support position i, giving e directly.
+    static constexpr int LOG = (T + 1 > 128) ? 8 : 7;  // 2^LOG >= t + 1 coefficients
+    static constexpr int NP = (1 << LOG) / 64;         // positions per lane in phase A
+    static constexpr int NW = 1 << (M - 5);            // 32-point words per ciphertext
+    static constexpr int WPL = NW / 64;                // words per lane in phase B
+    static constexpr int WPLEAF = 1 << (M - 5 - LOG);  // words covered by one leaf
+    static_assert(TILE * (1 << LOG) * 2 <= kWaveLds * 4, "leaf rows must fit the wave's LDS");
+
+    template <int D, int LANEX, int PBX>
+    static __device__ __forceinline__ void radix_pass(uint32_t (&F)[NP][M], int lane, int s, int sel)
+    {
+        // XOR into targets (bits s-1, s-2 of the node-local index == sel) the values at p ^ (LANEX + 64 PBX)
+#pragma unroll
+        for (int pb = 0; pb < NP; pb++) {
+            const int p = lane + 64 * pb;
+            const int i = p >> D;
+            const uint32_t hit = (((i >> (s - 2)) & 3) == sel) ? ~0u : 0u;
+            if ((pb ^ PBX) >= NP) continue;
+#pragma unroll
+            for (int k = 0; k < M; k++) F[pb][k] ^= xor_lanes_any<LANEX>(F[pb ^ PBX][k]) & hit;
+        }
+    }
+
+    // Radix sub-levels S .. 2 of depth D: pass 1 targets node-local bits (S-1, S-2) = (1,0) from
+    // distance q, pass 2 targets (0,1) from distance 3q; q = 2^(S-2) node-local = 2^(S-2+D) positions.
+    template <int D, int S>
+    static __device__ __forceinline__ void radix_levels(uint32_t (&F)[NP][M], int lane)
+    {
+        if constexpr (S >= 2) {
+            constexpr int Q = 1 << (S - 2 + D);
+            radix_pass<D, (Q & 63), (Q >> 6)>(F, lane, S, 2);
+            radix_pass<D, ((3 * Q) & 63), ((3 * Q) >> 6)>(F, lane, S, 1);
+            radix_levels<D, S - 1>(F, lane);
+        }
+    }
+
+    template <int D>
+    static __device__ __forceinline__ void coeff_depth(uint32_t (&F)[NP][M], int lane, const FftTab *tab)
+    {
+        if constexpr (D > 0) {
+#pragma unroll
+            for (int pb = 0; pb < NP; pb++) bs_mul_scalar<M>(F[pb], tab->tw[D][lane + 64 * pb]);
+        }
+        radix_levels<D, LOG - D>(F, lane);
+    }
+
+    static __device__ __forceinline__ void fft_coeff_phase(const uint32_t (&C)[NBL][M], uint32_t *lds, int lane,
+                                                           const FftTab *tab)
+    {
+        // C planes to LDS rows (row i = coefficient i), then gather f_p = C_{T-p}
+        wave_sync();
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+            const int i = lane + 64 * b;
+            if (i > T) continue;
+#pragma unroll
+            for (int k = 0; k < M; k++) lds[i * kRowWords + k] = C[b][k];
+        }
+        wave_sync();
+        uint32_t F[NP][M];
+#pragma unroll
+        for (int pb = 0; pb < NP; pb++) {
+            const int p = lane + 64 * pb;
+            const bool live = p <= T;
+            const int row = live ? T - p : 0;
+#pragma unroll
+            for (int k = 0; k < M; k++) F[pb][k] = live ? lds[row * kRowWords + k] : 0u;
+        }
+        wave_sync();
+        coeff_depth<0>(F, lane, tab);
+        coeff_depth<1>(F, lane, tab);
+        coeff_depth<2>(F, lane, tab);
+        coeff_depth<3>(F, lane, tab);
+        coeff_depth<4>(F, lane, tab);
+        coeff_depth<5>(F, lane, tab);
+        coeff_depth<6>(F, lane, tab);
+        if constexpr (LOG == 8) coeff_depth<7>(F, lane, tab);
+        // leaves per ciphertext: leaf[x][p]
+        uint16_t *leaf = reinterpret_cast<uint16_t *>(lds);
+#pragma unroll
+        for (int pb = 0; pb < NP; pb++) {
+            const int p = lane + 64 * pb;
+#pragma unroll 2
+            for (int x = 0; x < TILE; x++) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int k = 0; k < M; k++) v |= ((F[pb][k] >> x) & 1u) << k;
+                leaf[x * (1 << LOG) + p] = (uint16_t)v;
+            }
+        }
+        wave_sync();
+    }
+
+    // Butterfly level d: word pairs differ in word bit J = M - 6 - d.
+    template <int Dl>
+    static __device__ __forceinline__ void butterfly(uint32_t (&W)[WPL][M], int lane, const FftTab *tab)
+    {
+        constexpr int J = M - 6 - Dl;
+        constexpr int HW = 1 << J;  // words per half
+        uint32_t lowp[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
+        if constexpr (J >= 6) {
+            constexpr int RB = 1 << (J - 6);
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                if (r & RB) continue;
+                const int w = lane + 64 * r;
+                const uint32_t hv = tab->hi[Dl][w & (HW - 1)];
+                uint32_t G[M], t[M];
+#pragma unroll
+                for (int k = 0; k < M; k++) G[k] = lowp[k] ^ bitmask(hv, k);
+                bs_mul<M>(G, W[r | RB], t);
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    W[r][k] ^= t[k];
+                    W[r | RB][k] ^= W[r][k];
+                }
+            }
+        } else {
+            const bool upper = (lane >> J) & 1;
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                const int w = lane + 64 * r;
+                const uint32_t hv = tab->hi[Dl][w & (HW - 1)];
+                uint32_t G[M], u[M], v[M], t[M];
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    const uint32_t other = xor_lanes<(1 << J)>(W[r][k]);
+                    u[k] = upper ? other : W[r][k];
+                    v[k] = upper ? W[r][k] : other;
+                    G[k] = lowp[k] ^ bitmask(hv, k);
+                }
+                G[0] ^= upper ? ~0u : 0u;  // upper half: w1 = u + (G + 1) v
+                bs_mul<M>(G, v, t);
+#pragma unroll
+                for (int k = 0; k < M; k++) W[r][k] = u[k] ^ t[k];
+            }
+        }
+    }
+
+    template <int K>
+    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], const uint32_t *masks, int lane)
+    {
+        constexpr int S = K < M ? K : 2 * M - 2 - K;
+        uint32_t mw[WPL];
+#pragma unroll
+        for (int r = 0; r < WPL; r++) mw[r] = masks[K * NW + lane + 64 * r];
+        if constexpr (S < 5) {
+            constexpr int SH = 1 << S;
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                const uint32_t t = ((z[r] >> SH) ^ z[r]) & mw[r];
+                z[r] ^= t ^ (t << SH);
+            }
+        } else if constexpr (S - 5 >= 6) {
+            constexpr int RB = 1 << (S - 11);
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                if (r & RB) continue;
+                const uint32_t d = (z[r] ^ z[r | RB]) & mw[r];
+                z[r] ^= d;
+                z[r | RB] ^= d;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                const uint32_t other = xor_lanes<(1 << (S - 5))>(z[r]);
+                z[r] ^= (z[r] ^ other) & mw[r];
+            }
+        }
+    }
+
+    template <int... Ks>
+    static __device__ __forceinline__ void benes_all(uint32_t (&z)[WPL], const uint32_t *masks, int lane,
+                                                     std::integer_sequence<int, Ks...>)
+    {
+        (benes_layer<Ks>(z, masks, lane), ...);
+    }
+
+    static __device__ __forceinline__ void roots_fft(const FusedArgs &a, const FusedTile &tl, const uint32_t *lds,
+                                                     int lane, bool e_aligned, const FftTab *tab)
+    {
+        const uint16_t *leaf = reinterpret_cast<const uint16_t *>(lds);
+        for (int x = 0; x < tl.count; x++) {
+            const int item = a.items[tl.first + x];
+            const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
+            uint8_t *eo = a.e + (size_t)item * a.e_stride;
+            uint32_t W[WPL][M];
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                const int w = lane + 64 * r;
+                const uint32_t v = leaf[x * (1 << LOG) + bitrev_bits((uint32_t)(w / WPLEAF), LOG)];
+#pragma unroll
+                for (int k = 0; k < M; k++) W[r][k] = bitmask(v, k);
+            }
+            if constexpr (LOG == 8) butterfly<7>(W, lane, tab);
+            butterfly<6>(W, lane, tab);
+            butterfly<5>(W, lane, tab);
+            butterfly<4>(W, lane, tab);
+            butterfly<3>(W, lane, tab);
+            butterfly<2>(W, lane, tab);
+            butterfly<1>(W, lane, tab);
+            butterfly<0>(W, lane, tab);
+            uint32_t z[WPL];
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                uint32_t nz = 0;
+#pragma unroll
+                for (int k = 0; k < M; k++) nz |= W[r][k];
+                z[r] = ~nz;
+            }
+            benes_all(z, kd->benes, lane, std::make_integer_sequence<int, 2 * M - 1>{});
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                const int g = lane + 64 * r;
+                if (g >= G) continue;
+                const int nbytes = (N / 8 - 4 * g) < 4 ? (N / 8 - 4 * g) : 4;
+                uint8_t *dst = eo + 4 * g;
+                if (nbytes == 4 && e_aligned) {
+                    *reinterpret_cast<uint32_t *>(dst) = z[r];
+                } else {
+                    for (int q = 0; q < nbytes; q++) dst[q] = (uint8_t)(z[r] >> (8 * q));
+                }
+            }
+        }
+    }
+
+    static __device__ __forceinline__ void run(const FusedArgs &a, const FusedTile &tl, uint32_t *lds, int lane, bool e_aligned)
+    {
+        bool one_key = true;
+        {
+            const int k0 = a.item_key[tl.first];
+            for (int x = lane; x < tl.count; x += 64) one_key &= (a.item_key[tl.first + x] == k0);
+            one_key = __all(one_key);
+        }
+        syndrome(a, tl, one_key, lds, lane);
+        wave_sync();
+        uint32_t C[NBL][M];
+        berlekamp_massey(lds, lane, C);
+        if constexpr (ALG == 0) {
+            write_coef_rows(C, lds, lane);
+            roots(a, tl, lds, lane, e_aligned);
+        } else {
+            const FftTab *tab = a.fft + fft_variant(M, LOG);
+            fft_coeff_phase(C, lds, lane, tab);
+            roots_fft(a, tl, lds, lane, e_aligned, tab);
+        }
+        wave_sync();
+    }
+};
+
+
+// Persistent kernel: each wavefront pulls tiles until the list is exhausted.  Dynamic LDS of
+// kWaveLds words per wavefront; the workgroup size (64..256) gives the wavefronts per group.
+template <int ALG>
+__global__ __launch_bounds__(256) void k_fused(FusedArgs a)
+{
+    extern __shared__ uint32_t lds_dyn[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t *lds = lds_dyn + wv * kWaveLds;
+    const bool e_aligned = ((a.e_stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(a.e) & 3u) == 0);
+    for (;;) {
+        uint32_t tid = 0;
+        if (lane == 0) tid = atomicAdd(a.tile_counter, 1u);
+        tid = (uint32_t)__builtin_amdgcn_readfirstlane((int)tid);
+        if (tid >= (uint32_t)a.n_tiles) break;
+        const FusedTile tl = a.tiles[tid];
+        switch (tl.pidx) {
+        case 0: FusedTileKernel<0, ALG>::run(a, tl, lds, lane, e_aligned); break;
+        case 1: FusedTileKernel<1, ALG>::run(a, tl, lds, lane, e_aligned); break;
+        case 2: FusedTileKernel<2, ALG>::run(a, tl, lds, lane, e_aligned); break;
+        case 3: FusedTileKernel<3, ALG>::run(a, tl, lds, lane, e_aligned); break;
+        case 4: FusedTileKernel<4, ALG>::run(a, tl, lds, lane, e_aligned); break;
+        default: break;
+        }
+    }
+}
+
+// Launches k_fused<ALG> with the schedule's workgroup size and persistent grid.
+template <int ALG>
+static hipError_t launch_fused_variant(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
+{
+    static bool configured = false;
+    const size_t lds_bytes = (size_t)s.waves_per_group * kWaveLds * 4;
+    if (!configured) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_fused<ALG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           4 * kWaveLds * 4);
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    int groups = n_cu * s.groups_per_cu;
+    const int max_groups = (a.n_tiles + s.waves_per_group - 1) / s.waves_per_group;
+    if (groups > max_groups) groups = max_groups;
+    if (groups < 1) groups = 1;
+    hipLaunchKernelGGL((k_fused<ALG>), dim3(groups), dim3(64 * s.waves_per_group), lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace cmd

@@ -1,0 +1,38 @@
+// Fused-kernel dispatch and the small helper kernel the launch sequence uses.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace cmd {
+
+hipError_t launch_fused_fft(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+hipError_t launch_fused_horner(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+
+// Zero fill used instead of hipMemsetAsync so the launch sequence can be graph-captured.
+__global__ void k_zero_u32(uint32_t *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+bool fused_supported(const FusedSchedule &s)
+{
+    const bool tile_ok = s.tile_cts == 16 || s.tile_cts == 32;
+    const bool wpg_ok = s.waves_per_group == 1 || s.waves_per_group == 2 || s.waves_per_group == 4;
+    return tile_ok && wpg_ok && s.waves_per_ct == 1 && (s.root_alg == 0 || s.root_alg == 1);
+}
+
+hipError_t launch_fused(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
+{
+    if (!fused_supported(s)) return hipErrorInvalidValue;
+    if (a.n_tiles <= 0) return hipSuccess;
+    return s.root_alg == 1 ? launch_fused_fft(a, s, n_cu, st) : launch_fused_horner(a, s, n_cu, st);
+}
+
+}  // namespace cmd

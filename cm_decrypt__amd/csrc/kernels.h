@@ -12,9 +12,21 @@ struct KeyDev {
     const uint16_t *table;  // sb x 2t: L_i^j / g(L_i)^2
     const uint16_t *L;      // n support elements
     const uint32_t *lbits;  // m x lbits_stride bit planes of L (32 positions per word)
+    const uint32_t *benes;  // (2m-1) x 2^(m-5) swap masks of the key's Benes network, per 32-bit word
     int32_t pidx;
     int32_t lstride;        // words per bit-plane row
 };
+
+// Constants of the additive FFT that evaluates a polynomial of 2^l coefficients at all 2^m
+// field elements (evaluation index c holds the value at bitrev_m(c)).  They depend only on
+// (m, l); see fft_tables.cpp for their definition.
+struct FftTab {
+    uint16_t tw[8][256];      // twist at depth d for position p: lambda_d^(p >> d)
+    uint16_t hi[8][128];      // butterfly multiplier, high part, by word offset in the half
+    uint32_t lowpat[8][16];   // butterfly multiplier, low part, bit-sliced over the 32 bits of a word
+};
+constexpr int kFftVariants = 3;  // (m, l) = (12, 7), (13, 7), (13, 8)
+constexpr int fft_variant(int m, int l) { return m == 12 ? 0 : (l == 7 ? 1 : 2); }
 
 // ---- plain per-stage pipeline (one key per launch group) ----------------------------
 // Items k < count use ciphertext ct + idx[k]*ct_stride and write e + idx[k]*e_stride;
@@ -45,14 +57,19 @@ struct FusedArgs {
     uint8_t *e;
     uint64_t ct_stride, e_stride;
     uint32_t *tile_counter;    // work-queue head, zeroed before each launch
+    const FftTab *fft;         // kFftVariants tables
 };
 
 struct FusedSchedule {
-    int tile_cts;         // 8, 16 or 32
-    int waves_per_group;  // 1..8
-    int waves_per_ct;     // 1, 2 or 4
+    int tile_cts;         // 16 or 32
+    int waves_per_group;  // 1, 2 or 4
+    int waves_per_ct;     // 1
     int groups_per_cu;    // persistent workgroups per CU
+    int root_alg;         // 0: Horner at each support point, 1: additive FFT + Benes network
 };
+
+// Host: fills the FFT constants for (m, l).
+void build_fft_tab(int m, int l, FftTab *out);
 
 // Variant of the fused kernel compiled for a schedule, or false if not built.
 bool fused_supported(const FusedSchedule &s);

@@ -65,6 +65,22 @@ int expand_key(int pidx, const uint8_t *sk, size_t sk_len, ExpandedKey &out)
         for (int j = 1; j < w; j++) row[j] = F.mul(row[j - 1], out.L[i]);
     }
 
+    // Benes swap masks per word: layer k (stride 2^s), switch j acts on (c, c + 2^s)
+    const int words = 1 << (p.m - 5);
+    const uint8_t *cond = sk + 40 + 2 * p.t;
+    out.benes.assign((size_t)(2 * p.m - 1) * words, 0);
+    for (int k = 0; k < 2 * p.m - 1; k++) {
+        const int sft = k < p.m ? k : 2 * p.m - 2 - k;
+        const uint8_t *bits = cond + (size_t)k * ((size_t)1 << (p.m - 4));
+        for (int c = 0; c < (1 << p.m); c++) {
+            if ((c >> sft) & 1) continue;
+            const int j = ((c >> (sft + 1)) << sft) | (c & ((1 << sft) - 1));
+            if (!((bits[j >> 3] >> (j & 7)) & 1)) continue;
+            out.benes[(size_t)k * words + (c >> 5)] |= 1u << (c & 31);
+            if (sft >= 5) out.benes[(size_t)k * words + ((c + (1 << sft)) >> 5)] |= 1u << (c & 31);
+        }
+    }
+
     const int stride = lbits_stride(pidx);
     out.lbits.assign((size_t)p.m * stride, 0);
     for (int i = 0; i < p.n; i++)

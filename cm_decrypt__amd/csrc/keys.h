@@ -27,6 +27,8 @@ struct ExpandedKey {
     std::vector<uint16_t> L;      // n support elements
     std::vector<uint16_t> table;  // sb x 2t: table[i][j] = L_i^j / g(L_i)^2 (util.c:86-100)
     std::vector<uint32_t> lbits;  // m x groups: bit b of word [k][gi] = bit k of L[32*gi + b]
+    std::vector<uint32_t> benes;  // (2m-1) x 2^(m-5): per layer and 32-bit word, the positions whose
+                                  // switch is set (both words of a pair carry the pair's mask)
 };
 
 // initialisation + compute_inverses (CPU/param/param6960119/src/util.c:60-100) for one key.

@@ -129,6 +129,7 @@ int launch_slot(cmd_server *srv, Slot &s, std::vector<Req *> &batch)
     a.ct_stride = kCtStride;
     a.e_stride = kEStride;
     a.tile_counter = (uint32_t *)(d_plan + off_ctr);
+    a.fft = ctx->d_fft;
     CMD_HIP(launch_zero_u32(a.tile_counter, 1, s.stream));
     CMD_HIP(launch_fused(a, sched, ctx->n_cu, s.stream));
     CMD_HIP(hipMemcpyAsync(s.h_e, d_e, count * kEStride, hipMemcpyDeviceToHost, s.stream));

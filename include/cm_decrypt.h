@@ -129,12 +129,14 @@ int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes);
 
 /* Batch decryption, host buffers (copied in and out, blocking).  Item i uses key key_ids[i]
  * and its ciphertext at ct + i*ct_stride; its error vector (n/8 bytes, the reference's
- * build_error_vector, decrypt.c:43-53) goes to e_out + i*e_stride.  Sets and keys may mix. */
+ * build_error_vector, decrypt.c:43-53) goes to e_out + i*e_stride, and the rest of each
+ * e_stride row is zero.  Sets and keys may mix. */
 int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
                 const uint8_t *ct, size_t ct_stride, uint8_t *e_out, size_t e_stride);
 
 /* Same on device buffers already resident in HBM, enqueued on `stream` (a hipStream_t,
- * NULL = the context's stream); key_ids stay on the host.  Returns after enqueueing. */
+ * NULL = the context's stream); key_ids stay on the host.  Only the n/8 bytes of each row
+ * are written.  Returns once the work is complete. */
 int cmd_decrypt_device(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
                        const void *d_ct, size_t ct_stride, void *d_e, size_t e_stride, void *stream);
 
@@ -160,13 +162,17 @@ int cmd_stage_root(cmd_ctx *ctx, int32_t key_id, size_t count, const uint16_t *l
                    size_t e_stride);
 
 /* ---- schedules (autotuner) ----------------------------------------------------------- */
-/* Fused-kernel schedule per set: ciphertexts per wave-tile (8/16/32), waves per workgroup,
- * waves per ciphertext in the root stage.  The autotuner measures candidates on the GPU. */
+/* Fused-kernel schedule per set: ciphertexts per wavefront tile (16/32), waves per workgroup,
+ * waves per ciphertext in the root stage, persistent groups per CU, and the root-search
+ * algorithm (0: Horner at each support element as root_13.c:216-240 does; 1: additive FFT
+ * over all 2^m field elements followed by the key's Benes network).  Both give the same
+ * error vectors; the autotuner measures candidates on the GPU. */
 typedef struct {
     int32_t tile_cts;        /* ciphertexts per wavefront tile (bit-sliced BM width) */
     int32_t waves_per_group; /* wavefronts per workgroup */
     int32_t waves_per_ct;    /* wavefronts sharing one ciphertext's root search */
     int32_t groups_per_cu;   /* persistent workgroups per compute unit */
+    int32_t root_alg;        /* 0 Horner, 1 additive FFT + Benes */
 } cmd_schedule;
 
 int cmd_schedule_get(cmd_ctx *ctx, int32_t param_id, cmd_schedule *out);

@@ -128,3 +128,35 @@ def test_weight_below_t(ctx, keys, pid):
     for mode in (cm.MODE_PLAIN, cm.MODE_FUSED):
         got = ctx.decrypt([kid] * len(ct), ct, mode=mode)
         assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("alg", [0, 1])
+@pytest.mark.parametrize("tile,wpg", [(16, 1), (32, 2), (32, 4)])
+def test_fused_schedules_agree(ctx, keys, alg, tile, wpg):
+    """Every fused-kernel variant (Horner or FFT root search, tile width, wavefronts per group)
+    gives the oracle's error vectors on a mixed batch."""
+    rows, kids, want = [], [], []
+    for pid in PIDS:
+        sk, kid = keys[pid]
+        ct, _ = inputs(pid, sk, 9, 3, 500 + pid % 97)
+        w = orc.decrypt(pid, sk, ct)
+        for c, x in zip(ct, w):
+            r = np.zeros(208, np.uint8)
+            r[:c.size] = c
+            rows.append(r)
+            kids.append(kid)
+            wx = np.zeros(1024, np.uint8)
+            wx[:x.size] = x
+            want.append(wx)
+    saved = {pid: ctx.schedule(pid) for pid in PIDS}
+    try:
+        for pid in PIDS:
+            ctx.set_schedule(pid, tile_cts=tile, waves_per_group=wpg, root_alg=alg)
+        got = ctx.decrypt(np.array(kids, np.int32), np.stack(rows), mode=cm.MODE_FUSED, ct_stride=208, e_stride=1024)
+    finally:
+        for pid, s in saved.items():
+            ctx.set_schedule(pid, **s)
+    want = np.stack(want)
+    bad = [i for i in range(len(rows)) if not np.array_equal(got[i], want[i])]
+    detail = [(i, int(np.nonzero(got[i] != want[i])[0][0])) for i in bad[:6]]
+    assert not bad, f"{len(bad)} rows differ (row, first byte): {detail}"

@@ -160,8 +160,7 @@ def test_cli_argument_and_file_errors():
 
 def test_no_device_is_reported(monkeypatch):
     """Creating a context without a usable GPU fails with CMD_ERR_NODEV (no CPU fallback)."""
-    import torch
-    if torch.cuda.device_count() > 0:
+    if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present")
     with pytest.raises(cm.CmdError) as ei:
         cm.Context(0)
