@@ -14,6 +14,7 @@ Prints one JSON line.  Only one GPU is used; --gpus above 1 is refused.
 import argparse
 import concurrent.futures as cf
 import json
+import re
 import os
 import sys
 import time
@@ -134,7 +135,7 @@ def mixed_leg(cm, torch, ctx, pool, n_items, keys_per_set, steps, seed, threads)
             "mix": {str(pid): counts[pid] for pid, _ in MIX}}
 
 
-def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads):
+def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads, inflight):
     """Poisson arrivals of the 5-set mix over the resident key pool; p50/p99 per rate."""
     rng = np.random.default_rng(seed)
     per_set = {pid: len(keys) for pid, keys in pool.items()}
@@ -154,43 +155,67 @@ def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads):
         f"({time.perf_counter() - t0:.1f}s setup)")
     weights = np.array([dict(MIX)[p[1]] / per_set[p[1]] for p in pool])
     weights /= weights.sum()
+    t_kid = np.array([p[0] for p in pool], np.int32)
+    t_ct = np.zeros((len(pool), 208), np.uint8)
+    t_e = np.zeros((len(pool), 1024), np.uint8)
+    for i, (_, _, c, x) in enumerate(pool):
+        t_ct[i, :c.size] = c
+        t_e[i, :x.size] = x
     out = []
-    srv = cm.Server(ctx, max_batch=8192, max_inflight=3, min_batch=1, max_wait_us=0.0)
+    srv = cm.Server(ctx, max_batch=8192, max_inflight=inflight, min_batch=1, max_wait_us=0.0)
     try:
         for rate in rates:
             n_req = max(200, int(rate * duration_s))
             picks = rng.choice(len(pool), size=n_req, p=weights)
-            gaps = rng.exponential(1.0 / rate, size=n_req)
-            arrivals = np.cumsum(gaps)
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+            kid_seq, ct_seq = t_kid[picks], t_ct[picks]
             srv.reset_stats()
-            ids = []
+            rids = np.zeros(n_req, np.uint64)
             start = time.perf_counter()
-            for i in range(n_req):
-                target = start + arrivals[i]
-                while True:
-                    now = time.perf_counter()
-                    if now >= target:
-                        break
-                    if target - now > 2e-4:
-                        time.sleep((target - now) * 0.5)
-                kid, pid, c, _ = pool[picks[i]]
-                ids.append(srv.submit(kid, c))
-            ok = True
-            for i, rid in enumerate(ids):
-                kid, pid, c, x = pool[picks[i]]
-                got = srv.wait(rid, x.size, 30e6)
-                ok &= bool(np.array_equal(got, x))
+            i = 0
+            while i < n_req:
+                # submit every request whose Poisson arrival time has passed, in one call
+                now = time.perf_counter() - start
+                j = int(np.searchsorted(arrivals, now, side="right"))
+                if j > i:
+                    rids[i:j] = srv.submit_batch(kid_seq[i:j], ct_seq[i:j], 208)
+                    i = j
+                else:
+                    gap = arrivals[i] - now
+                    if gap > 2e-4:
+                        time.sleep(gap * 0.5)
+            got = srv.wait_batch(rids, 1024, 60e6)
             elapsed = time.perf_counter() - start
+            ok = bool(np.array_equal(got, t_e[picks]))
             st = srv.stats()
             out.append({"offered_req_per_s": rate, "requests": n_req, "dec_per_s": round(n_req / elapsed, 1),
                         "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
                         "max_us": round(st["max_us"], 1), "mean_batch": round(st["mean_batch"], 1),
                         "launches": st["launches"], "bit_exact": ok})
-            log(f"serve rate {rate:.0f}/s: p50 {st['p50_us']:.0f}us p99 {st['p99_us']:.0f}us "
-                f"mean batch {st['mean_batch']:.1f} exact={ok}")
+            log(f"serve rate {rate:.0f}/s: {n_req / elapsed:,.0f} dec/s p50 {st['p50_us']:.0f}us "
+                f"p99 {st['p99_us']:.0f}us mean batch {st['mean_batch']:.1f} exact={ok}")
     finally:
         srv.close()
     return out
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_8192128_16keys_65536_pmc.md")
+
+
+def pmc_traffic(sched):
+    """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (already scaled by the copy-kernel calibration), if it was taken with this schedule."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None, None
+    vals = dict(re.findall(r"^(traffic_bytes_per_launch|valu_insts_per_launch|schedule):\s*(.+)$", text,
+                           flags=re.M))
+    if "schedule" in vals and json.loads(vals["schedule"]) != sched:
+        return None, None
+    tr = int(float(vals["traffic_bytes_per_launch"])) if "traffic_bytes_per_launch" in vals else None
+    vi = float(vals["valu_insts_per_launch"]) if "valu_insts_per_launch" in vals else None
+    return tr, vi
 
 
 def cpu_baseline(cm, sks, ct, key_of, budget_s):
@@ -234,9 +259,12 @@ def main():
     ap.add_argument("--serve-keys", type=int, default=1024)
     ap.add_argument("--serve-rates", default="1e4,1e5,1e6")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--threads", type=int, default=16)
-    ap.add_argument("--schedules", default=os.path.join(ROOT, "cm_decrypt__amd", "schedules.json"))
+    ap.add_argument("--schedules", default=os.path.join(ROOT, "cm_decrypt__amd", "schedules.json"),
+                    help="tuned schedules loaded with --no-autotune")
+    ap.add_argument("--save-schedules", default="", help="write the autotuned schedules here")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -277,6 +305,11 @@ def main():
         best, best_us = ctx.autotune(SET, key_ids, reps=2)
         tuned = {"schedule": best, "best_us": round(best_us, 1), "items": int(N_CT)}
         log(f"autotuned schedule {best} ({best_us:.0f} us for {N_CT} items)")
+        if args.save_schedules:
+            ctx.save_schedules(args.save_schedules)
+    elif os.path.exists(args.schedules):
+        ctx.load_schedules(args.schedules)
+        tuned = {"loaded": os.path.relpath(args.schedules, ROOT)}
     sched = ctx.schedule(SET)
 
     fused = cm.Batch(ctx, key_ids, p.ct_bytes, p.e_bytes, mode=cm.MODE_FUSED, use_graph=True)
@@ -312,13 +345,15 @@ def main():
                 f"exact={mixed['bit_exact']}")
         if not args.no_serve:
             rates = [float(x) for x in args.serve_rates.split(",") if x]
-            serve = serve_leg(cm, ctx, pool, rates, args.serve_seconds, 17, threads)
+            serve = serve_leg(cm, ctx, pool, rates, args.serve_seconds, 17, threads, args.serve_inflight)
 
     cpu = None if args.no_cpu else cpu_baseline(cm, sks, ct, key_of, args.cpu_seconds)
 
+    traffic, valu_insts = pmc_traffic(sched)
     kern_s = kern_ms * 1e-3
     ops = OPS_PER_DEC[SET] * N_CT
-    valu_peak = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+    nominal_peak = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+    valu_peak = ctx.valu_peak() or nominal_peak
     achieved_tops = ops / kern_s / 1e12
     hbm_gbs = BYTES_PER_DEC[SET] * N_CT / kern_s / 1e9
     line = {
@@ -343,13 +378,16 @@ def main():
         "fused_vs_plain": round(value / plain["value"], 3) if plain else None,
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(valu_peak / 1e12, 2),
                      "unit": "Tops/s (int32 lane-ops)", "frac": round(achieved_tops * 1e12 / valu_peak, 4),
-                     "traffic": None,
+                     "traffic": traffic,
                      "kernel_ms": round(kern_ms, 4),
-                     "note": "achieved = 13.28M int32-op equivalents per decryption (reference algorithm) x "
-                             "65536 / fused-kernel time (HIP events on the launch stream); peak = "
-                             "256 CU x 64 lanes x 2.4 GHz"},
+                     "peak_nominal": round(nominal_peak / 1e12, 2),
+                     "issue_frac": (round(valu_insts * 64 / kern_s / valu_peak, 4) if valu_insts else None),
+                     "note": "achieved = 13.28M int32-op equivalents per decryption of the reference algorithm "
+                             "(Horner root search) x 65536 / fused launch time (HIP events on the launch stream); "
+                             "the kernel's additive FFT needs fewer operations, so frac can exceed 1; peak = "
+                             "measured v_xor_b32 issue rate of this GPU (nominal 256 CU x 64 lanes x 2.4 GHz)"},
         "roofline_hbm": {"bound": "hbm", "achieved": round(hbm_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "traffic": None},
+                         "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "mixed_batch": mixed,
         "serve": serve,
         "cpu_baseline": cpu,

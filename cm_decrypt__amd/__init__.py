@@ -146,6 +146,8 @@ _sig("cmd_schedule_save", _I, _P, _c.c_char_p)
 _sig("cmd_schedule_load", _I, _P, _c.c_char_p)
 _sig("cmd_server_create", _I, _P, _c.POINTER(ServerConfig), _c.POINTER(_P))
 _sig("cmd_server_submit", _I, _P, _I32, _P, _c.POINTER(_U64))
+_sig("cmd_server_submit_batch", _I, _P, _SZ, _P, _P, _SZ, _P, _c.POINTER(_SZ))
+_sig("cmd_server_wait_batch", _I, _P, _SZ, _P, _P, _SZ, _c.c_double)
 _sig("cmd_server_poll", _I, _P, _U64, _P)
 _sig("cmd_server_wait", _I, _P, _U64, _P, _c.c_double)
 _sig("cmd_server_stats_get", _I, _P, _c.POINTER(ServerStats))
@@ -153,6 +155,7 @@ _sig("cmd_server_reset_stats", _I, _P)
 _sig("cmd_server_destroy", _I, _P)
 _sig("cmd_set_kernel_timing", _I, _P, _I32)
 _sig("cmd_kernel_timing", _I, _P, _c.POINTER(_c.c_double), _c.POINTER(_U64))
+_sig("cmd_valu_peak", _I, _P, _c.POINTER(_c.c_double))
 _sig("cmd_build_info", _c.c_char_p)
 
 lib = _lib
@@ -193,6 +196,13 @@ def build_info() -> str:
 
 
 # ---- host-only helpers -------------------------------------------------------------------
+
+def key_device_bytes(param_id: int) -> int:
+    """Device bytes one resident key of this parameter set occupies."""
+    b = _U64(0)
+    _check(_lib.cmd_key_device_bytes(param_id, _c.byref(b)), "cmd_key_device_bytes")
+    return int(b.value)
+
 
 def support_gen(param_id: int, cond: np.ndarray) -> np.ndarray:
     p = params(param_id)
@@ -332,7 +342,10 @@ class Context:
         if ct_stride is None:
             ct_stride = ct.shape[1] if ct.ndim == 2 else ct.size // max(count, 1)
         if e_stride is None:
-            e_stride = max(params(self.key_param[int(k)]).e_bytes for k in np.unique(keys)) if count else 0
+            # keys this object did not load (or unloaded) get the widest row; the library
+            # itself rejects unknown keys with CMD_ERR_KEY
+            e_stride = max((params(self.key_param[int(k)]).e_bytes if int(k) in self.key_param else 1024)
+                           for k in np.unique(keys)) if count else 0
         e = np.zeros((count, e_stride), np.uint8)
         _check(_lib.cmd_decrypt(self._h, mode, count, _ptr(keys), _ptr(ct), ct_stride, _ptr(e), e_stride),
                "cmd_decrypt")
@@ -396,6 +409,12 @@ class Context:
     def set_kernel_timing(self, on: bool):
         _check(_lib.cmd_set_kernel_timing(self._h, int(on)), "cmd_set_kernel_timing")
 
+    def valu_peak(self) -> float:
+        """Measured VALU issue rate (32-bit lane operations per second)."""
+        v = _c.c_double(0)
+        _check(_lib.cmd_valu_peak(self._h, _c.byref(v)), "cmd_valu_peak")
+        return float(v.value)
+
     def kernel_timing(self):
         us, n = _c.c_double(0), _U64(0)
         _check(_lib.cmd_kernel_timing(self._h, _c.byref(us), _c.byref(n)), "cmd_kernel_timing")
@@ -446,6 +465,26 @@ class Server:
         rid = _U64(0)
         _check(_lib.cmd_server_submit(self._h, key_id, _ptr(ct), _c.byref(rid)), "cmd_server_submit")
         return int(rid.value)
+
+    def submit_batch(self, key_ids, ct: np.ndarray, ct_stride: Optional[int] = None) -> np.ndarray:
+        """Submits len(key_ids) requests (rows of ct); returns their ids."""
+        keys = np.ascontiguousarray(key_ids, dtype=np.int32)
+        ct = np.ascontiguousarray(ct, dtype=np.uint8)
+        if ct_stride is None:
+            ct_stride = ct.shape[1] if ct.ndim == 2 else ct.size // max(keys.size, 1)
+        rids = np.zeros(keys.size, np.uint64)
+        acc = _SZ(0)
+        _check(_lib.cmd_server_submit_batch(self._h, keys.size, _ptr(keys), _ptr(ct), ct_stride, _ptr(rids),
+                                            _c.byref(acc)), "cmd_server_submit_batch")
+        return rids
+
+    def wait_batch(self, rids: np.ndarray, e_stride: int = 1024, timeout_us: float = -1.0) -> np.ndarray:
+        """Error vectors of the requests rids (rows of e_stride bytes)."""
+        rids = np.ascontiguousarray(rids, dtype=np.uint64)
+        e = np.zeros((rids.size, e_stride), np.uint8)
+        _check(_lib.cmd_server_wait_batch(self._h, rids.size, _ptr(rids), _ptr(e), e_stride, timeout_us),
+               "cmd_server_wait_batch")
+        return e
 
     def try_submit(self, key_id: int, ct: np.ndarray):
         ct = np.ascontiguousarray(ct, dtype=np.uint8)

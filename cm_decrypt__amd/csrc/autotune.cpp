@@ -23,7 +23,7 @@ static std::vector<FusedSchedule> candidates()
                 for (int gpc : {1, 2, 4, 8}) {
                     FusedSchedule s{tile, wpg, 1, gpc, alg};
                     // LDS per workgroup is wpg * 16.4 KB; keep resident groups within 160 KB per CU
-                    if (wpg * gpc > 9) continue;
+                    if (wpg * gpc > 8) continue;  // at most two wavefronts per SIMD
                     if (fused_supported(s)) out.push_back(s);
                 }
     return out;

@@ -115,24 +115,45 @@ __device__ __forceinline__ uint32_t gf_inv(uint32_t a)
 }
 
 // ---- bit-sliced ------------------------------------------------------------------------
+// gfx950's three-input v_bitop3_b32 evaluates any function of three operands in one
+// instruction; its 8-bit table is indexed by (src0 << 2) | (src1 << 1) | src2.
+// 0x6A: src2 ^ (src0 & src1)      0x96: src0 ^ src1 ^ src2
+// Written as builtins so that every AND-XOR term of a product is one instruction (left to
+// itself the compiler re-associates the sums into trees of separate ANDs and XORs).
+__device__ __forceinline__ uint32_t and_xor(uint32_t acc, uint32_t x, uint32_t y)
+{
+    return __builtin_amdgcn_bitop3_b32(x, y, acc, 0x6A);
+}
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// Folds planes 2M-2 .. M of an unreduced product back into planes 0 .. M-1.
+template <int M>
+__device__ __forceinline__ void bs_fold_planes(uint32_t p[2 * M - 1])
+{
+    constexpr uint32_t low = Modulus<M>::low;
+#pragma unroll
+    for (int s = 2 * M - 2; s >= M; s--)
+#pragma unroll
+        for (int b2 = 0; b2 < 5; b2++)
+            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+}
 
 // out = a * b for 32 element pairs (bit lanes).  out may alias a or b.
 template <int M>
 __device__ __forceinline__ void bs_mul(const uint32_t a[M], const uint32_t b[M], uint32_t out[M])
 {
-    constexpr uint32_t low = Modulus<M>::low;
     uint32_t p[2 * M - 1];
 #pragma unroll
     for (int s = 0; s < 2 * M - 1; s++) p[s] = 0;
 #pragma unroll
     for (int i = 0; i < M; i++)
 #pragma unroll
-        for (int j = 0; j < M; j++) p[i + j] ^= a[i] & b[j];
-#pragma unroll
-    for (int s = 2 * M - 2; s >= M; s--)
-#pragma unroll
-        for (int b2 = 0; b2 < 5; b2++)
-            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+        for (int j = 0; j < M; j++) p[i + j] = and_xor(p[i + j], a[i], b[j]);
+    bs_fold_planes<M>(p);
 #pragma unroll
     for (int k = 0; k < M; k++) out[k] = p[k];
 }
@@ -142,19 +163,17 @@ template <int M>
 __device__ __forceinline__ void bs_mul_add2(const uint32_t a[M], const uint32_t b[M], const uint32_t c[M],
                                             const uint32_t d[M], uint32_t out[M])
 {
-    constexpr uint32_t low = Modulus<M>::low;
     uint32_t p[2 * M - 1];
 #pragma unroll
     for (int s = 0; s < 2 * M - 1; s++) p[s] = 0;
 #pragma unroll
     for (int i = 0; i < M; i++)
 #pragma unroll
-        for (int j = 0; j < M; j++) p[i + j] ^= (a[i] & b[j]) ^ (c[i] & d[j]);
-#pragma unroll
-    for (int s = 2 * M - 2; s >= M; s--)
-#pragma unroll
-        for (int b2 = 0; b2 < 5; b2++)
-            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+        for (int j = 0; j < M; j++) {
+            p[i + j] = and_xor(p[i + j], a[i], b[j]);
+            p[i + j] = and_xor(p[i + j], c[i], d[j]);
+        }
+    bs_fold_planes<M>(p);
 #pragma unroll
     for (int k = 0; k < M; k++) out[k] = p[k];
 }
@@ -166,19 +185,14 @@ __device__ __forceinline__ void bs_mac_wide(const uint32_t a[M], const uint32_t 
 #pragma unroll
     for (int i = 0; i < M; i++)
 #pragma unroll
-        for (int j = 0; j < M; j++) p[i + j] ^= a[i] & b[j];
+        for (int j = 0; j < M; j++) p[i + j] = and_xor(p[i + j], a[i], b[j]);
 }
 
 // Folds an unreduced product back to m planes.
 template <int M>
 __device__ __forceinline__ void bs_fold(uint32_t p[2 * M - 1], uint32_t out[M])
 {
-    constexpr uint32_t low = Modulus<M>::low;
-#pragma unroll
-    for (int s = 2 * M - 2; s >= M; s--)
-#pragma unroll
-        for (int b2 = 0; b2 < 5; b2++)
-            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+    bs_fold_planes<M>(p);
 #pragma unroll
     for (int k = 0; k < M; k++) out[k] = p[k];
 }

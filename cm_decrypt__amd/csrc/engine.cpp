@@ -96,6 +96,8 @@ static int upload_key_slot(cmd_ctx *ctx, int32_t id)
 
 void batch_release(cmd_batch *b)
 {
+    for (int32_t k : b->pinned) b->ctx->keys[k].inflight--;
+    b->pinned.clear();
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -550,6 +552,11 @@ static int make_batch(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *k
     if (rc) {
         batch_release(b.get());
         return rc;
+    }
+    // a captured graph holds the keys' device pointers: keep them loaded while the batch lives
+    for (const auto &grp : b->groups) {
+        ctx->keys[grp.key].inflight++;
+        b->pinned.push_back(grp.key);
     }
     *out = b.release();
     return CMD_OK;

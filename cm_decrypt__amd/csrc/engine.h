@@ -37,7 +37,7 @@ constexpr size_t kMaxKeys = 16384;  // resident key slots per context
 struct KeySlot {
     bool used = false;
     int pidx = -1;
-    int inflight = 0;      // server requests referencing the key (guarded by the context lock)
+    int inflight = 0;      // server requests and live batches referencing the key (context lock)
     KeyDev dev{};
     void *mem = nullptr;   // one allocation holding table, L, lbits
     size_t bytes = 0;
@@ -88,6 +88,7 @@ struct cmd_batch {
     std::vector<cmd::FusedTile> tiles;
     std::vector<int32_t> items, item_key;
     int launch_tile = 32;      // kernel tile width: the widest tile of the batch
+    std::vector<int32_t> pinned;  // distinct keys of the batch, held against unload
     cmd::DevBuf d_plan;       // device copy of idx / tiles / items / keys
     int32_t *d_idx = nullptr; // plain: item order (grouped)
     cmd::FusedTile *d_tiles = nullptr;

@@ -1,0 +1,11 @@
+// Fused kernel, additive-FFT root search (root_alg 1), built for two wavefronts per SIMD (256 registers).
+#include "fused_impl.h"
+
+namespace cmd {
+
+hipError_t launch_fused_fft_occ2(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
+{
+    return launch_fused_variant<1, 2>(a, s, n_cu, st);
+}
+
+}  // namespace cmd

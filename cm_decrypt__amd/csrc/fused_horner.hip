@@ -1,11 +1,11 @@
-// Fused kernel, Horner root search at each support element (root_alg 0).
+// Fused kernel, Horner root search at each support element (root_alg 0), built for one wavefront per SIMD.
 #include "fused_impl.h"
 
 namespace cmd {
 
-hipError_t launch_fused_horner(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
+hipError_t launch_fused_horner_occ1(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
 {
-    return launch_fused_variant<0>(a, s, n_cu, st);
+    return launch_fused_variant<0, 1>(a, s, n_cu, st);
 }
 
 }  // namespace cmd

@@ -216,8 +216,10 @@ struct FusedTileKernel {
                     }
                 }
             } else {
-                // one key per ciphertext: each row load feeds one accumulator, so issue a whole
-                // ciphertext word's 32 rows at once to keep many loads in flight
+                // one key per ciphertext: each row feeds one accumulator.  The ciphertext is
+                // public, so only the rows of its set bits are read (a wave-uniform bit scan),
+                // GRP rows at a time to keep loads in flight.
+                constexpr int GRP = 16;
 #pragma unroll
                 for (int x = 0; x < CH; x++) {
                     if (x0 + x >= tl.count) continue;
@@ -229,28 +231,23 @@ struct FusedTileKernel {
 #pragma unroll
                         for (int b = 0; b < 4; b++)
                             if (4 * w + b < Pm::synd_bytes) cwv |= (uint32_t)ctp[4 * w + b] << (8 * b);
-                        const uint32_t cw = uni(cwv);
-                        if (w * 32 + 32 <= Pm::sb) {
-                            uint32_t v[32][R];
+                        uint32_t bits = uni(cwv);
+                        const uint32_t *wrows = rows + (size_t)(w * 32) * PAIRS;
+                        while (bits) {
+                            uint32_t v[GRP][R];
 #pragma unroll
-                            for (int bit = 0; bit < 32; bit++)
-#pragma unroll
-                                for (int r = 0; r < R; r++)
-                                    v[bit][r] = (lane + 64 * r < PAIRS) ? rows[(size_t)(w * 32 + bit) * PAIRS + lane + 64 * r] : 0u;
-#pragma unroll
-                            for (int bit = 0; bit < 32; bit++) {
-                                const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw, bit, 1);
-#pragma unroll
-                                for (int r = 0; r < R; r++) acc[x][r] ^= v[bit][r] & msk;
-                            }
-                        } else {
-                            for (int bit = 0; bit < Pm::sb - w * 32; bit++) {
-                                const uint32_t *row = rows + (size_t)(w * 32 + bit) * PAIRS;
-                                const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw, bit, 1);
+                            for (int j = 0; j < GRP; j++) {
+                                const bool any = bits != 0;
+                                const int bit = any ? __builtin_ctz(bits) : 0;
+                                bits &= bits - 1;
 #pragma unroll
                                 for (int r = 0; r < R; r++)
-                                    acc[x][r] ^= ((lane + 64 * r < PAIRS) ? row[lane + 64 * r] : 0u) & msk;
+                                    v[j][r] = (any && lane + 64 * r < PAIRS) ? wrows[(size_t)bit * PAIRS + lane + 64 * r] : 0u;
                             }
+#pragma unroll
+                            for (int j = 0; j < GRP; j++)
+#pragma unroll
+                                for (int r = 0; r < R; r++) acc[x][r] ^= v[j][r];
                         }
                     }
                 }
@@ -541,6 +538,9 @@ struct FusedTileKernel {
     {
         constexpr int J = M - 6 - Dl;
         constexpr int HW = 1 << J;  // words per half
+        // Reload this level's constants here: hoisted out of the per-ciphertext loop, the
+        // multipliers of all levels would not fit in registers and would go to scratch.
+        asm volatile("" ::: "memory");
         uint32_t lowp[M];
 #pragma unroll
         for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
@@ -560,6 +560,9 @@ struct FusedTileKernel {
                     W[r][k] ^= t[k];
                     W[r | RB][k] ^= W[r][k];
                 }
+                // one word pair at a time: interleaving the products would multiply the
+                // live registers
+                __builtin_amdgcn_sched_barrier(0);
             }
         } else {
             const bool upper = (lane >> J) & 1;
@@ -579,6 +582,7 @@ struct FusedTileKernel {
                 bs_mul<M>(G, v, t);
 #pragma unroll
                 for (int k = 0; k < M; k++) W[r][k] = u[k] ^ t[k];
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
@@ -697,8 +701,10 @@ struct FusedTileKernel {
 
 // Persistent kernel: each wavefront pulls tiles until the list is exhausted.  Dynamic LDS of
 // kWaveLds words per wavefront; the workgroup size (64..256) gives the wavefronts per group.
-template <int ALG>
-__global__ __launch_bounds__(256) void k_fused(FusedArgs a)
+// OCC is the number of wavefronts per SIMD the build is register-limited for: 1 (up to 512
+// VGPR+AGPR per lane) or 2 (256).
+template <int ALG, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC))) void k_fused(FusedArgs a)
 {
     extern __shared__ uint32_t lds_dyn[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -721,15 +727,15 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs a)
     }
 }
 
-// Launches k_fused<ALG> with the schedule's workgroup size and persistent grid.
-template <int ALG>
+// Launches k_fused<ALG, OCC> with the schedule's workgroup size and persistent grid.
+template <int ALG, int OCC>
 static hipError_t launch_fused_variant(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
 {
     static bool configured = false;
     const size_t lds_bytes = (size_t)s.waves_per_group * kWaveLds * 4;
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_fused<ALG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           4 * kWaveLds * 4);
+        hipError_t e = hipFuncSetAttribute((const void *)k_fused<ALG, OCC>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 4 * kWaveLds * 4);
         if (e != hipSuccess) return e;
         configured = true;
     }
@@ -737,7 +743,7 @@ static hipError_t launch_fused_variant(const FusedArgs &a, const FusedSchedule &
     const int max_groups = (a.n_tiles + s.waves_per_group - 1) / s.waves_per_group;
     if (groups > max_groups) groups = max_groups;
     if (groups < 1) groups = 1;
-    hipLaunchKernelGGL((k_fused<ALG>), dim3(groups), dim3(64 * s.waves_per_group), lds_bytes, st, a);
+    hipLaunchKernelGGL((k_fused<ALG, OCC>), dim3(groups), dim3(64 * s.waves_per_group), lds_bytes, st, a);
     return hipGetLastError();
 }
 

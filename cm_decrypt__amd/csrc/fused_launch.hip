@@ -5,8 +5,10 @@
 
 namespace cmd {
 
-hipError_t launch_fused_fft(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
-hipError_t launch_fused_horner(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+hipError_t launch_fused_fft_occ1(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+hipError_t launch_fused_fft_occ2(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+hipError_t launch_fused_horner_occ1(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+hipError_t launch_fused_horner_occ2(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
 
 // Zero fill used instead of hipMemsetAsync so the launch sequence can be graph-captured.
 __global__ void k_zero_u32(uint32_t *p, int n)
@@ -32,7 +34,10 @@ hipError_t launch_fused(const FusedArgs &a, const FusedSchedule &s, int n_cu, hi
 {
     if (!fused_supported(s)) return hipErrorInvalidValue;
     if (a.n_tiles <= 0) return hipSuccess;
-    return s.root_alg == 1 ? launch_fused_fft(a, s, n_cu, st) : launch_fused_horner(a, s, n_cu, st);
+    // more than one wavefront per SIMD needs the 256-register build
+    const bool occ2 = s.waves_per_group * s.groups_per_cu > 4;
+    if (s.root_alg == 1) return occ2 ? launch_fused_fft_occ2(a, s, n_cu, st) : launch_fused_fft_occ1(a, s, n_cu, st);
+    return occ2 ? launch_fused_horner_occ2(a, s, n_cu, st) : launch_fused_horner_occ1(a, s, n_cu, st);
 }
 
 }  // namespace cmd

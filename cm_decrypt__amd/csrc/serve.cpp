@@ -331,6 +331,61 @@ extern "C" int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *c
     return CMD_OK;
 }
 
+extern "C" int cmd_server_submit_batch(cmd_server *s, size_t count, const int32_t *key_ids, const uint8_t *ct,
+                                       size_t ct_stride, uint64_t *reqs, size_t *accepted)
+{
+    if (!s || (count && (!key_ids || !ct || !reqs))) return CMD_ERR_ARG;
+    size_t done = 0;
+    int rc = CMD_OK;
+    {
+        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+        for (size_t i = 0; i < count; i++) {
+            const int32_t k = key_ids[i];
+            if (k < 0 || (size_t)k >= s->ctx->keys.size() || !s->ctx->keys[k].used) {
+                if (accepted) *accepted = 0;
+                return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)k);
+            }
+            if (ct_stride < (size_t)kParams[s->ctx->keys[k].pidx].synd_bytes) {
+                if (accepted) *accepted = 0;
+                return set_error(CMD_ERR_ARG, "ct_stride %zu below the ciphertext size of item %zu", ct_stride, i);
+            }
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return set_error(CMD_ERR_ARG, "server is shutting down");
+        const double t = now_us();
+        for (; done < count; done++) {
+            if ((int)s->pending.size() >= s->cfg.queue_capacity) {
+                s->rejected += count - done;
+                rc = set_error(CMD_ERR_FULL, "%d requests pending", (int)s->pending.size());
+                break;
+            }
+            Req *r = new (std::nothrow) Req();
+            if (!r) {
+                rc = CMD_ERR_NOMEM;
+                break;
+            }
+            r->id = s->next_id++;
+            r->key = key_ids[done];
+            {
+                std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+                r->pidx = s->ctx->keys[r->key].pidx;
+                s->ctx->keys[r->key].inflight++;
+            }
+            std::memcpy(r->ct, ct + done * ct_stride, kParams[r->pidx].synd_bytes);
+            r->t_submit = t;
+            s->reqs[r->id] = r;
+            s->pending.push_back(r);
+            s->submitted++;
+            reqs[done] = r->id;
+        }
+    }
+    if (accepted) *accepted = done;
+    s->cv_work.notify_one();
+    return rc;
+}
+
 static int take_result(cmd_server *s, uint64_t id, uint8_t *e_out)
 {
     auto it = s->reqs.find(id);
@@ -366,6 +421,38 @@ extern "C" int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *e_out, doub
             return rc == CMD_ERR_PENDING ? CMD_ERR_TIMEOUT : rc;
         }
     }
+}
+
+extern "C" int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t *reqs, uint8_t *e_out,
+                                     size_t e_stride, double timeout_us)
+{
+    if (!s || (count && (!reqs || !e_out))) return CMD_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    const auto deadline = Clock::now() + std::chrono::microseconds((long long)(timeout_us < 0 ? 0 : timeout_us));
+    for (size_t i = 0; i < count; i++) {
+        auto it = s->reqs.find(reqs[i]);
+        if (it == s->reqs.end()) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)reqs[i]);
+        if ((size_t)kParams[it->second->pidx].n / 8 > e_stride)
+            return set_error(CMD_ERR_ARG, "e_stride %zu below the error vector of request %zu", e_stride, i);
+    }
+    for (size_t i = 0; i < count; i++) {
+        for (;;) {
+            int rc = take_result(s, reqs[i], e_out + i * e_stride);
+            if (rc != CMD_ERR_PENDING) {
+                if (rc) return rc;
+                break;
+            }
+            if (timeout_us < 0) {
+                s->cv_done.wait(lk);
+            } else if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+                rc = take_result(s, reqs[i], e_out + i * e_stride);
+                if (rc == CMD_ERR_PENDING) return CMD_ERR_TIMEOUT;
+                if (rc) return rc;
+                break;
+            }
+        }
+    }
+    return CMD_OK;
 }
 
 extern "C" int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out)

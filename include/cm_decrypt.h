@@ -207,10 +207,19 @@ typedef struct {
 int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd_server **out);
 /* Copies the ciphertext (ct_bytes of the key's set); *req receives the request id. */
 int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *ct, uint64_t *req);
+/* Submits count requests at once (item i: key_ids[i], ciphertext at ct + i*ct_stride);
+ * reqs[i] receives the ids.  Keys and strides are checked before anything is queued; if the
+ * queue fills, the first *accepted requests stay submitted and CMD_ERR_FULL is returned. */
+int cmd_server_submit_batch(cmd_server *s, size_t count, const int32_t *key_ids, const uint8_t *ct,
+                            size_t ct_stride, uint64_t *reqs, size_t *accepted);
 /* CMD_OK and the error vector once done, CMD_ERR_PENDING before; frees the request. */
 int cmd_server_poll(cmd_server *s, uint64_t req, uint8_t *e_out);
 /* Blocks until done or timeout_us elapses (< 0: no limit). */
 int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *e_out, double timeout_us);
+/* Waits for count requests and writes their error vectors to e_out + i*e_stride (the
+ * requests are freed); CMD_ERR_TIMEOUT once timeout_us (< 0: no limit) has passed. */
+int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t *reqs, uint8_t *e_out, size_t e_stride,
+                          double timeout_us);
 int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out);
 int cmd_server_reset_stats(cmd_server *s);
 /* Stops admission, drains in-flight work, frees everything. */
@@ -222,6 +231,11 @@ int cmd_server_destroy(cmd_server *s);
  * stream), and resets the accumulator. */
 int cmd_set_kernel_timing(cmd_ctx *ctx, int32_t on);
 int cmd_kernel_timing(cmd_ctx *ctx, double *avg_us, uint64_t *launches);
+
+/* Measured VALU issue rate of the context's device in 32-bit lane operations per second
+ * (chains of single v_xor_b32 instructions over all CUs): the peak the decoder's integer
+ * work is compared with.  No counterpart in the reference; a measurement aid. */
+int cmd_valu_peak(cmd_ctx *ctx, double *lane_ops_per_s);
 
 /* Library build identification (kernel variants compiled in). */
 const char *cmd_build_info(void);

@@ -10,7 +10,7 @@
  * CPU/param/param<p>/src/util.c, CPU/param/param6960119/src/decrypt.c) for a
  * parameter set chosen at run time.  Citations sit beside each function.
  *
- * Parity pins (tests/test_oracle_*.py):
+ * Parity pins (tests/test_oracle.py):
  *   orc_gf_mul      exhaustive vs. a carry-less textbook product + reduction,
  *                   field axioms, and vs. the reference build (oracle/_ref).
  *   orc_gf_inv/frac a*inv(a)==1 for all nonzero a, inv(0)==0, vs. oracle/_ref.

@@ -131,10 +131,10 @@ def test_weight_below_t(ctx, keys, pid):
 
 
 @pytest.mark.parametrize("alg", [0, 1])
-@pytest.mark.parametrize("tile,wpg", [(16, 1), (32, 2), (32, 4)])
-def test_fused_schedules_agree(ctx, keys, alg, tile, wpg):
-    """Every fused-kernel variant (Horner or FFT root search, tile width, wavefronts per group)
-    gives the oracle's error vectors on a mixed batch."""
+@pytest.mark.parametrize("tile,wpg,gpc", [(16, 1, 2), (32, 2, 2), (32, 4, 1), (32, 1, 8), (16, 2, 4), (32, 4, 2)])
+def test_fused_schedules_agree(ctx, keys, alg, tile, wpg, gpc):
+    """Every fused-kernel variant (Horner or FFT root search, tile width, wavefronts per group,
+    one- or two-wavefront-per-SIMD build) gives the oracle's error vectors on a mixed batch."""
     rows, kids, want = [], [], []
     for pid in PIDS:
         sk, kid = keys[pid]
@@ -151,7 +151,7 @@ def test_fused_schedules_agree(ctx, keys, alg, tile, wpg):
     saved = {pid: ctx.schedule(pid) for pid in PIDS}
     try:
         for pid in PIDS:
-            ctx.set_schedule(pid, tile_cts=tile, waves_per_group=wpg, root_alg=alg)
+            ctx.set_schedule(pid, tile_cts=tile, waves_per_group=wpg, groups_per_cu=gpc, root_alg=alg)
         got = ctx.decrypt(np.array(kids, np.int32), np.stack(rows), mode=cm.MODE_FUSED, ct_stride=208, e_stride=1024)
     finally:
         for pid, s in saved.items():
