@@ -168,6 +168,32 @@ struct FusedTileKernel {
     static constexpr int CST = (T + 2) & ~1;             // LDS uint16 stride of a coefficient row
     static_assert(TILE * CST * 2 <= kWaveLds * 4, "coefficient rows must fit the wave's LDS");
 
+    // Eight table rows (bits 8G .. 8G+7 of a ciphertext word) into the accumulators of the CH
+    // ciphertexts of a single-key chunk.  All eight rows' loads are issued before the first
+    // use; the per-ciphertext bit masks are wave-uniform and made together, which lets the
+    // compiler keep them in SGPRs (scalar s_bfe) so each term costs one v_bitop3.
+    template <int G>
+    static __device__ __forceinline__ void syndrome_rows(const uint32_t *wrows, const uint32_t (&cw)[CH],
+                                                         uint32_t (&acc)[CH][R], int lane)
+    {
+        uint32_t v[8][R];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                v[q][r] = (lane + 64 * r < PAIRS) ? wrows[(size_t)(8 * G + q) * PAIRS + lane + 64 * r] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            uint32_t msk[CH];
+#pragma unroll
+            for (int x = 0; x < CH; x++) msk[x] = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x], 8 * G + q, 1);
+#pragma unroll
+            for (int x = 0; x < CH; x++)
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[x][r] = and_xor(acc[x][r], v[q][r], msk[x]);
+        }
+    }
+
     // ---- stage 1: syndromes into LDS bit planes ---------------------------------------------
     static __device__ __forceinline__ void syndrome(const FusedArgs &a, const FusedTile &tl, bool one_key, uint32_t *lds,
                                     int lane)
@@ -200,18 +226,25 @@ struct FusedTileKernel {
                     uint32_t cw[CH];
 #pragma unroll
                     for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
-                    const int nbits = (w * 32 + 32 <= Pm::sb) ? 32 : (Pm::sb - w * 32);
-#pragma unroll 8
-                    for (int bit = 0; bit < nbits; bit++) {
-                        const uint32_t *row = rows + (size_t)(w * 32 + bit) * PAIRS;
-                        uint32_t v[R];
+                    const uint32_t *wrows = rows + (size_t)(w * 32) * PAIRS;
+                    if (w * 32 + 32 <= Pm::sb) {
+                        syndrome_rows<0>(wrows, cw, acc, lane);
+                        syndrome_rows<1>(wrows, cw, acc, lane);
+                        syndrome_rows<2>(wrows, cw, acc, lane);
+                        syndrome_rows<3>(wrows, cw, acc, lane);
+                    } else {
+                        // partial last word (6960119: 16 bits)
+                        for (int bit = 0; bit < Pm::sb - w * 32; bit++) {
+                            uint32_t v[R];
 #pragma unroll
-                        for (int r = 0; r < R; r++) v[r] = (lane + 64 * r < PAIRS) ? row[lane + 64 * r] : 0u;
+                            for (int r = 0; r < R; r++)
+                                v[r] = (lane + 64 * r < PAIRS) ? wrows[(size_t)bit * PAIRS + lane + 64 * r] : 0u;
 #pragma unroll
-                        for (int x = 0; x < CH; x++) {
-                            const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x], bit, 1);
+                            for (int x = 0; x < CH; x++) {
+                                const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x], bit, 1);
 #pragma unroll
-                            for (int r = 0; r < R; r++) acc[x][r] ^= v[r] & msk;
+                                for (int r = 0; r < R; r++) acc[x][r] = and_xor(acc[x][r], v[r], msk);
+                            }
                         }
                     }
                 }
