@@ -69,6 +69,15 @@ __device__ __forceinline__ uint32_t shift_up1(uint32_t v, uint32_t carry)
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// The same value, hidden from the optimiser.  Values derived from the lane id inside a loop
+// would otherwise be hoisted to the kernel entry, where hundreds of them no longer fit in
+// registers and are spilled; deriving them from an opaque copy keeps them where they are used.
+__device__ __forceinline__ int opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // Bit-sliced "L <= K" over 9-bit values L (planes Lp), K a public constant.
 __device__ __forceinline__ uint32_t bs_le_const(const uint32_t Lp[9], uint32_t K)
 {
@@ -164,9 +173,10 @@ struct FusedTileKernel {
     static constexpr int PAIRS = T;                      // table dwords per row (2t uint16)
     static constexpr int R = (PAIRS + 63) / 64;
     static constexpr int WORDS = (Pm::synd_bytes + 3) / 4;
-    static constexpr int CH = TILE < 16 ? TILE : 16;     // ciphertexts per syndrome chunk
+    static constexpr int CH = TILE;                      // ciphertexts per syndrome chunk
     static constexpr int CST = (T + 2) & ~1;             // LDS uint16 stride of a coefficient row
     static_assert(TILE * CST * 2 <= kWaveLds * 4, "coefficient rows must fit the wave's LDS");
+    static_assert(CH == TILE && CH % 16 == 0, "one syndrome chunk covers the tile");
 
     // Eight table rows (bits 8G .. 8G+7 of a ciphertext word) into the accumulators of the CH
     // ciphertexts of a single-key chunk.  All eight rows' loads are issued before the first
@@ -184,13 +194,17 @@ struct FusedTileKernel {
                 v[q][r] = (lane + 64 * r < PAIRS) ? wrows[(size_t)(8 * G + q) * PAIRS + lane + 64 * r] : 0u;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            uint32_t msk[CH];
+            // sixteen masks at a time bounds the SGPRs they hold
 #pragma unroll
-            for (int x = 0; x < CH; x++) msk[x] = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x], 8 * G + q, 1);
+            for (int x0 = 0; x0 < CH; x0 += 16) {
+                uint32_t msk[16];
 #pragma unroll
-            for (int x = 0; x < CH; x++)
+                for (int x = 0; x < 16; x++) msk[x] = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x0 + x], 8 * G + q, 1);
 #pragma unroll
-                for (int r = 0; r < R; r++) acc[x][r] = and_xor(acc[x][r], v[q][r], msk[x]);
+                for (int x = 0; x < 16; x++)
+#pragma unroll
+                    for (int r = 0; r < R; r++) acc[x0 + x][r] = and_xor(acc[x0 + x][r], v[q][r], msk[x]);
+            }
         }
     }
 
@@ -299,12 +313,7 @@ struct FusedTileKernel {
                         uint32_t plane = 0;
 #pragma unroll
                         for (int x = 0; x < CH; x++) plane |= ((acc[x][r] >> (16 * h + k)) & 1u) << x;
-                        uint32_t *dst = &lds[(j + 1) * kRowWords + k];
-                        if constexpr (TILE == 32) {
-                            reinterpret_cast<uint16_t *>(dst)[c] = (uint16_t)plane;
-                        } else {
-                            *dst = plane;
-                        }
+                        lds[(j + 1) * kRowWords + k] = plane;
                     }
                 }
             }
@@ -565,68 +574,154 @@ struct FusedTileKernel {
         wave_sync();
     }
 
-    // Butterfly level d: word pairs differ in word bit J = M - 6 - d.
+    // ---- phase B data layout --------------------------------------------------------------
+    // Word w (32 evaluation points) of a ciphertext lives in lane l, register r.  Its index bits
+    // start as w = l + 64 r.  Butterfly level J pairs words that differ in index bit J; when that
+    // bit is a lane bit, it is first exchanged with a register bit (a 2 x 2 transpose between
+    // lane pairs), so every butterfly runs inside one lane on half the words: no product is
+    // computed twice.  The exchange schedule is fixed at compile time: when bit J is in a lane,
+    // it replaces the register bit whose index is needed furthest in the future (or never).
+    static constexpr int RBITS = WPL == 4 ? 2 : 1;      // register bits of the word index
+    static constexpr int NLEV = LOG;                    // butterfly levels J = JMIN .. JMIN+LOG-1
+    static constexpr int JMIN = M - 5 - LOG;
+    struct Layout {
+        int posr[2];   // word-index bit held by register bit q
+        int posl[6];   // word-index bit held by lane bit i
+    };
+    struct Schedule {
+        int swap_lane[8];   // lane bit exchanged with register bit swap_q[i] before level i (-1: none)
+        int swap_q[8];
+        int bq[8];          // register bit holding bit J at level i
+        Layout at[8];       // layout during level i
+        Layout fin;         // layout after the last level
+    };
+    static constexpr Schedule make_schedule()
+    {
+        Schedule sc{};
+        Layout cur{};
+        for (int q = 0; q < 2; q++) cur.posr[q] = q < RBITS ? 6 + q : -1;
+        for (int i = 0; i < 6; i++) cur.posl[i] = i;
+        for (int i = 0; i < NLEV; i++) {
+            const int J = JMIN + i;
+            int q_hold = -1;
+            for (int q = 0; q < RBITS; q++)
+                if (cur.posr[q] == J) q_hold = q;
+            sc.swap_lane[i] = -1;
+            sc.swap_q[i] = -1;
+            if (q_hold < 0) {
+                int lb = 0;
+                for (int b = 0; b < 6; b++)
+                    if (cur.posl[b] == J) lb = b;
+                int best = 0, key = -1;
+                for (int q = 0; q < RBITS; q++) {
+                    const int k = cur.posr[q] < J ? 1000 : cur.posr[q];
+                    if (k > key) { key = k; best = q; }
+                }
+                sc.swap_lane[i] = lb;
+                sc.swap_q[i] = best;
+                cur.posl[lb] = cur.posr[best];
+                cur.posr[best] = J;
+                q_hold = best;
+            }
+            sc.bq[i] = q_hold;
+            sc.at[i] = cur;
+        }
+        sc.fin = cur;
+        return sc;
+    }
+    static constexpr Schedule SCHED = make_schedule();
+
+    // Index bits of lane l's words that come from the lane: bits of l placed where the layout
+    // says, restricted to word-index bits below LIMIT.
+    static __device__ __forceinline__ uint32_t lane_part(const Layout &lay, int lane, int limit)
+    {
+        uint32_t w = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+            if (lay.posl[i] < limit) w |= (uint32_t)((lane >> i) & 1) << lay.posl[i];
+        return w;
+    }
+    static constexpr uint32_t reg_part(const Layout &lay, int r, int limit)
+    {
+        uint32_t w = 0;
+        for (int q = 0; q < RBITS; q++)
+            if (lay.posr[q] < limit) w |= (uint32_t)((r >> q) & 1) << lay.posr[q];
+        return w;
+    }
+
+    // Exchanges lane bit LB with register bit Q: lane l keeps the words whose register bit Q
+    // equals its lane bit LB and trades the others with lane l ^ 2^LB.
+    template <int LB, int Q>
+    static __device__ __forceinline__ void swap_bits(uint32_t (&W)[WPL][M], int lane)
+    {
+        const bool upper = (opaque(lane) >> LB) & 1;
+#pragma unroll
+        for (int r0 = 0; r0 < WPL; r0++) {
+            if ((r0 >> Q) & 1) continue;
+            const int r1 = r0 | (1 << Q);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const uint32_t send = upper ? W[r0][k] : W[r1][k];
+                const uint32_t recv = xor_lanes<(1 << LB)>(send);
+                W[r0][k] = upper ? recv : W[r0][k];
+                W[r1][k] = upper ? W[r1][k] : recv;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // Butterfly level Dl (word-index bit J = M - 6 - Dl): w0 = u + G v, w1 = w0 + v, where
+    // G = lowpat (by bit position) + hi (by the low J bits of the lower word's index).
     template <int Dl>
     static __device__ __forceinline__ void butterfly(uint32_t (&W)[WPL][M], int lane, const FftTab *tab)
     {
         constexpr int J = M - 6 - Dl;
-        constexpr int HW = 1 << J;  // words per half
+        constexpr int LI = J - JMIN;
+        constexpr int SL = SCHED.swap_lane[LI], SQ = SCHED.swap_q[LI];
+        if constexpr (SL >= 0) swap_bits<(SL >= 0 ? SL : 0), (SQ >= 0 ? SQ : 0)>(W, lane);
+        constexpr int RB = 1 << SCHED.bq[LI];
+        constexpr Layout LAY = SCHED.at[LI];
         // Reload this level's constants here: hoisted out of the per-ciphertext loop, the
         // multipliers of all levels would not fit in registers and would go to scratch.
         asm volatile("" ::: "memory");
         uint32_t lowp[M];
 #pragma unroll
         for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
-        if constexpr (J >= 6) {
-            constexpr int RB = 1 << (J - 6);
+        const uint32_t wl = lane_part(LAY, opaque(lane), J);
 #pragma unroll
-            for (int r = 0; r < WPL; r++) {
-                if (r & RB) continue;
-                const int w = lane + 64 * r;
-                const uint32_t hv = tab->hi[Dl][w & (HW - 1)];
-                uint32_t G[M], t[M];
+        for (int r = 0; r < WPL; r++) {
+            if (r & RB) continue;
+            const uint32_t hv = tab->hi[Dl][wl | reg_part(LAY, r, J)];
+            uint32_t G[M], t[M];
 #pragma unroll
-                for (int k = 0; k < M; k++) G[k] = lowp[k] ^ bitmask(hv, k);
-                bs_mul<M>(G, W[r | RB], t);
+            for (int k = 0; k < M; k++) G[k] = lowp[k] ^ bitmask(hv, k);
+            bs_mul<M>(G, W[r | RB], t);
 #pragma unroll
-                for (int k = 0; k < M; k++) {
-                    W[r][k] ^= t[k];
-                    W[r | RB][k] ^= W[r][k];
-                }
-                // one word pair at a time: interleaving the products would multiply the
-                // live registers
-                __builtin_amdgcn_sched_barrier(0);
+            for (int k = 0; k < M; k++) {
+                W[r][k] ^= t[k];
+                W[r | RB][k] ^= W[r][k];
             }
-        } else {
-            const bool upper = (lane >> J) & 1;
-#pragma unroll
-            for (int r = 0; r < WPL; r++) {
-                const int w = lane + 64 * r;
-                const uint32_t hv = tab->hi[Dl][w & (HW - 1)];
-                uint32_t G[M], u[M], v[M], t[M];
-#pragma unroll
-                for (int k = 0; k < M; k++) {
-                    const uint32_t other = xor_lanes<(1 << J)>(W[r][k]);
-                    u[k] = upper ? other : W[r][k];
-                    v[k] = upper ? W[r][k] : other;
-                    G[k] = lowp[k] ^ bitmask(hv, k);
-                }
-                G[0] ^= upper ? ~0u : 0u;  // upper half: w1 = u + (G + 1) v
-                bs_mul<M>(G, v, t);
-#pragma unroll
-                for (int k = 0; k < M; k++) W[r][k] = u[k] ^ t[k];
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            // one word pair at a time: interleaving the products would multiply the live
+            // registers
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 
+    // Full word index of lane l's register r in the final layout.
+    static __device__ __forceinline__ uint32_t final_word(uint32_t wl_fin, int r)
+    {
+        return wl_fin | reg_part(SCHED.fin, r, 64);
+    }
+
     template <int K>
-    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], const uint32_t *masks, int lane)
+    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], const uint32_t *masks, int lane,
+                                                       uint32_t wl_fin)
     {
         constexpr int S = K < M ? K : 2 * M - 2 - K;
+        const uint32_t *mrow = masks + K * NW + opaque((int)wl_fin);
         uint32_t mw[WPL];
 #pragma unroll
-        for (int r = 0; r < WPL; r++) mw[r] = masks[K * NW + lane + 64 * r];
+        for (int r = 0; r < WPL; r++) mw[r] = mrow[reg_part(SCHED.fin, r, 64)];
         if constexpr (S < 5) {
             constexpr int SH = 1 << S;
 #pragma unroll
@@ -634,29 +729,36 @@ struct FusedTileKernel {
                 const uint32_t t = ((z[r] >> SH) ^ z[r]) & mw[r];
                 z[r] ^= t ^ (t << SH);
             }
-        } else if constexpr (S - 5 >= 6) {
-            constexpr int RB = 1 << (S - 11);
-#pragma unroll
-            for (int r = 0; r < WPL; r++) {
-                if (r & RB) continue;
-                const uint32_t d = (z[r] ^ z[r | RB]) & mw[r];
-                z[r] ^= d;
-                z[r | RB] ^= d;
-            }
         } else {
+            constexpr int JB = S - 5;  // word-index bit
+            constexpr int QB = SCHED.fin.posr[0] == JB ? 0 : (RBITS > 1 && SCHED.fin.posr[1] == JB ? 1 : -1);
+            if constexpr (QB >= 0) {
+                constexpr int RB = 1 << QB;
 #pragma unroll
-            for (int r = 0; r < WPL; r++) {
-                const uint32_t other = xor_lanes<(1 << (S - 5))>(z[r]);
-                z[r] ^= (z[r] ^ other) & mw[r];
+                for (int r = 0; r < WPL; r++) {
+                    if (r & RB) continue;
+                    const uint32_t d = (z[r] ^ z[r | RB]) & mw[r];
+                    z[r] ^= d;
+                    z[r | RB] ^= d;
+                }
+            } else {
+                constexpr int LBB = SCHED.fin.posl[0] == JB ? 0 : SCHED.fin.posl[1] == JB ? 1
+                                  : SCHED.fin.posl[2] == JB ? 2 : SCHED.fin.posl[3] == JB ? 3
+                                  : SCHED.fin.posl[4] == JB ? 4 : 5;
+#pragma unroll
+                for (int r = 0; r < WPL; r++) {
+                    const uint32_t other = xor_lanes<(1 << LBB)>(z[r]);
+                    z[r] ^= (z[r] ^ other) & mw[r];
+                }
             }
         }
     }
 
     template <int... Ks>
     static __device__ __forceinline__ void benes_all(uint32_t (&z)[WPL], const uint32_t *masks, int lane,
-                                                     std::integer_sequence<int, Ks...>)
+                                                     uint32_t wl_fin, std::integer_sequence<int, Ks...>)
     {
-        (benes_layer<Ks>(z, masks, lane), ...);
+        (benes_layer<Ks>(z, masks, lane, wl_fin), ...);
     }
 
     static __device__ __forceinline__ void roots_fft(const FusedArgs &a, const FusedTile &tl, const uint32_t *lds,
@@ -664,13 +766,14 @@ struct FusedTileKernel {
     {
         const uint16_t *leaf = reinterpret_cast<const uint16_t *>(lds);
         for (int x = 0; x < tl.count; x++) {
+            const uint32_t wl_fin = lane_part(SCHED.fin, opaque(lane), 64);
             const int item = a.items[tl.first + x];
             const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
             uint8_t *eo = a.e + (size_t)item * a.e_stride;
             uint32_t W[WPL][M];
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
-                const int w = lane + 64 * r;
+                const int w = lane + 64 * r;  // initial layout
                 const uint32_t v = leaf[x * (1 << LOG) + bitrev_bits((uint32_t)(w / WPLEAF), LOG)];
 #pragma unroll
                 for (int k = 0; k < M; k++) W[r][k] = bitmask(v, k);
@@ -691,10 +794,10 @@ struct FusedTileKernel {
                 for (int k = 0; k < M; k++) nz |= W[r][k];
                 z[r] = ~nz;
             }
-            benes_all(z, kd->benes, lane, std::make_integer_sequence<int, 2 * M - 1>{});
+            benes_all(z, kd->benes, lane, wl_fin, std::make_integer_sequence<int, 2 * M - 1>{});
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
-                const int g = lane + 64 * r;
+                const int g = (int)final_word(wl_fin, r);
                 if (g >= G) continue;
                 const int nbytes = (N / 8 - 4 * g) < 4 ? (N / 8 - 4 * g) : 4;
                 uint8_t *dst = eo + 4 * g;
