@@ -18,7 +18,7 @@ static std::vector<FusedSchedule> candidates()
 {
     std::vector<FusedSchedule> out;
     for (int alg : {1, 0})
-        for (int tile : {16, 32})
+        for (int tile : {4, 8, 16, 32})
             for (int wpg : {1, 2, 4})
                 for (int gpc : {1, 2, 4, 8}) {
                     FusedSchedule s{tile, wpg, 1, gpc, alg};

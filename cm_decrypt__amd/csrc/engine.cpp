@@ -802,6 +802,6 @@ extern "C" int cmd_kernel_timing(cmd_ctx *ctx, double *avg_us, uint64_t *launche
 
 extern "C" const char *cmd_build_info(void)
 {
-    return "cm_decrypt__amd gfx950: plain(synd/bm/chien, uint16 GF) fused(tile 16|32, waves/group 1|2|4, "
+    return "cm_decrypt__amd gfx950: plain(synd/bm/chien, uint16 GF) fused(tile 1..32, waves/group 1|2|4, one- or two-wave builds, "
            "bit-sliced BM + root search)";
 }

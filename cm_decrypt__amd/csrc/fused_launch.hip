@@ -25,7 +25,9 @@ hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t st)
 
 bool fused_supported(const FusedSchedule &s)
 {
-    const bool tile_ok = s.tile_cts == 16 || s.tile_cts == 32;
+    // the kernel is bit-sliced 32 wide; narrower tiles leave bit positions idle but spread a
+    // small batch over more wavefronts
+    const bool tile_ok = s.tile_cts >= 1 && s.tile_cts <= 32 && (s.tile_cts & (s.tile_cts - 1)) == 0;
     const bool wpg_ok = s.waves_per_group == 1 || s.waves_per_group == 2 || s.waves_per_group == 4;
     return tile_ok && wpg_ok && s.waves_per_ct == 1 && (s.root_alg == 0 || s.root_alg == 1);
 }

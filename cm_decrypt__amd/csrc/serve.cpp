@@ -99,8 +99,19 @@ int launch_slot(cmd_server *srv, Slot &s, std::vector<Req *> &batch)
     FusedSchedule sched;
     {
         std::lock_guard<std::recursive_mutex> g(ctx->lock);
-        plan_fused(ctx, ctx->sched, count, key_ids.data(), tiles, items, item_key, &launch_tile);
-        sched = launch_schedule(ctx->sched, tiles, launch_tile);
+        // Latency first: narrow the tiles until the launch has about one wavefront per SIMD
+        // (a tile's Berlekamp-Massey takes the same time for 1 or 32 ciphertexts, while its
+        // syndrome and root search grow with the count), never wider than the tuned width.
+        FusedSchedule per_set[kNumParams];
+        const int target = ctx->n_cu * 4;
+        for (int i = 0; i < kNumParams; i++) {
+            per_set[i] = ctx->sched[i];
+            int tile = per_set[i].tile_cts;
+            while (tile > 1 && (int)((count + tile - 1) / tile) < target) tile >>= 1;
+            per_set[i].tile_cts = tile;
+        }
+        plan_fused(ctx, per_set, count, key_ids.data(), tiles, items, item_key, &launch_tile);
+        sched = launch_schedule(per_set, tiles, launch_tile);
     }
     for (size_t i = 0; i < count; i++) std::memcpy(s.h_ct + i * kCtStride, batch[i]->ct, kParams[batch[i]->pidx].synd_bytes);
 

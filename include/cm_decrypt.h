@@ -168,7 +168,8 @@ int cmd_stage_root(cmd_ctx *ctx, int32_t key_id, size_t count, const uint16_t *l
  * over all 2^m field elements followed by the key's Benes network).  Both give the same
  * error vectors; the autotuner measures candidates on the GPU. */
 typedef struct {
-    int32_t tile_cts;        /* ciphertexts per wavefront tile (bit-sliced BM width) */
+    int32_t tile_cts;        /* ciphertexts per wavefront tile: 1, 2, 4, 8, 16 or 32 (the kernel is
+                                bit-sliced 32 wide; narrower tiles spread small batches wider) */
     int32_t waves_per_group; /* wavefronts per workgroup */
     int32_t waves_per_ct;    /* wavefronts sharing one ciphertext's root search */
     int32_t groups_per_cu;   /* persistent workgroups per compute unit */

@@ -243,7 +243,7 @@ def test_schedule_persistence_and_autotune(ctx, keys):
     saved = {q: ctx.schedule(q) for q in PIDS}
     try:
         best, us = ctx.autotune(pid, [kid] * 2048, reps=1)
-        assert us > 0 and best["tile_cts"] in (16, 32) and best["root_alg"] in (0, 1)
+        assert us > 0 and best["tile_cts"] in (1, 2, 4, 8, 16, 32) and best["root_alg"] in (0, 1)
         assert ctx.schedule(pid) == best
         ct, e = cm.gen_encrypt(pid, sk, 17, 40)
         assert np.array_equal(ctx.decrypt([kid] * 40, ct), e)

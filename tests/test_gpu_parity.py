@@ -131,7 +131,8 @@ def test_weight_below_t(ctx, keys, pid):
 
 
 @pytest.mark.parametrize("alg", [0, 1])
-@pytest.mark.parametrize("tile,wpg,gpc", [(16, 1, 2), (32, 2, 2), (32, 4, 1), (32, 1, 8), (16, 2, 4), (32, 4, 2)])
+@pytest.mark.parametrize("tile,wpg,gpc", [(16, 1, 2), (32, 2, 2), (32, 4, 1), (32, 1, 8), (16, 2, 4), (32, 4, 2),
+                                          (4, 2, 4), (1, 4, 1)])
 def test_fused_schedules_agree(ctx, keys, alg, tile, wpg, gpc):
     """Every fused-kernel variant (Horner or FFT root search, tile width, wavefronts per group,
     one- or two-wavefront-per-SIMD build) gives the oracle's error vectors on a mixed batch."""
