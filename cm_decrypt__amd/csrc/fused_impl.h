@@ -321,8 +321,10 @@ struct FusedTileKernel {
     }
 
     // ---- stage 2: bit-sliced Berlekamp-Massey; leaves coefficient rows in LDS ----------------
-    // Lane l holds coefficient i = l + 64b of C and B (b < NBL).  At iteration n only
-    // coefficients i <= n+1 can be nonzero, so block b joins once 64b <= n+1 (a public bound).
+    // Lane l holds coefficient i = l + 64b of C and B (b < NBL).  Before iteration n's update C
+    // has degree <= n and B degree <= n+1 (B = x^(n+1) until the first nonzero discrepancy), so
+    // block b joins the products once 64b <= n+1 and the shift of B once 64b <= n+2 (public
+    // bounds).
     static __device__ __forceinline__ void berlekamp_massey(uint32_t *lds, int lane, uint32_t (&C)[NBL][M])
     {
         uint32_t B[NBL][M];
@@ -383,10 +385,11 @@ struct FusedTileKernel {
                     C[b][k] = cn[k];
                 }
             }
-            // shift the highest block first so each carry is read before its source moves
+            // shift the highest block first so each carry is read before its source moves; B has
+            // degree <= n+1 here, so block b can receive a carry once 64b <= n+2
 #pragma unroll
             for (int b = NBL - 1; b >= 0; b--) {
-                if (64 * b > n + 1) continue;
+                if (64 * b > n + 2) continue;
 #pragma unroll
                 for (int k = 0; k < M; k++) {
                     const uint32_t carry =
@@ -403,6 +406,100 @@ struct FusedTileKernel {
             for (int q = 0; q < 9; q++) Lp[q] = (Ln[q] & mle) | (Lp[q] & ~mle);
 #pragma unroll
             for (int k = 0; k < M; k++) bb[k] = (d[k] & mle) | (bb[k] & ~mle);
+        }
+    }
+
+    // ---- stage 2 for tiles of at most SCALAR_BM ciphertexts --------------------------------
+    // The bit-sliced loop costs the same for 1 or 32 ciphertexts.  Small tiles (low-load
+    // serving) instead run the same inversion-free recurrence per ciphertext with scalar field
+    // elements, lane l holding coefficient l + 64b, and pack the result into the bit-sliced
+    // planes the root search reads (bit x of C[b][k] = bit k of ciphertext x's coefficient).
+    static constexpr int SCALAR_BM = 3;
+    static constexpr int NQ = (2 * T + 63) / 64;  // syndromes per lane
+
+    static __device__ __forceinline__ void berlekamp_massey_scalar(uint32_t *lds, int lane, int count,
+                                                                   uint32_t (&C)[NBL][M])
+    {
+        // scalar syndromes of the first SCALAR_BM slots out of the planes, then as uint16 rows
+        uint32_t sx[SCALAR_BM][NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int j = lane + 64 * q;
+            uint32_t pl[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) pl[k] = (j < 2 * T) ? lds[(j + 1) * kRowWords + k] : 0u;
+#pragma unroll
+            for (int x = 0; x < SCALAR_BM; x++) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int k = 0; k < M; k++) v |= ((pl[k] >> x) & 1u) << k;
+                sx[x][q] = v;
+            }
+        }
+        wave_sync();
+        uint16_t *S = reinterpret_cast<uint16_t *>(lds);
+#pragma unroll
+        for (int x = 0; x < SCALAR_BM; x++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int j = lane + 64 * q;
+                if (j < 2 * T) S[x * 2 * T + j] = (uint16_t)sx[x][q];
+            }
+        wave_sync();
+
+#pragma unroll
+        for (int b = 0; b < NBL; b++)
+#pragma unroll
+            for (int k = 0; k < M; k++) C[b][k] = 0;
+        const uint32_t keep_last = (lane + 64 * (NBL - 1) <= T) ? ~0u : 0u;
+        for (int x = 0; x < count; x++) {
+            const uint16_t *Sx = S + x * 2 * T;
+            uint32_t c[NBL], bv[NBL];
+#pragma unroll
+            for (int b = 0; b < NBL; b++) {
+                c[b] = (b == 0 && lane == 0) ? 1u : 0u;   // C = 1
+                bv[b] = (b == 0 && lane == 1) ? 1u : 0u;  // B = x
+            }
+            uint32_t bb = 1, L = 0;
+#pragma unroll 1
+            for (int n = 0; n < 2 * T; n++) {
+                uint32_t part = 0;
+#pragma unroll
+                for (int b = 0; b < NBL; b++) {
+                    if (64 * b > n + 1) continue;
+                    const int j = n - (lane + 64 * b);
+                    const uint32_t sv = j >= 0 ? (uint32_t)Sx[j] : 0u;
+                    part ^= gf_mul<M>(c[b], sv);
+                }
+                const uint32_t d = uni(wave_xor_all(part));
+                const bool mle = d != 0 && 2 * L <= (uint32_t)n;
+                uint32_t colb[M], cold[M];
+                gf_mul_cols<M>(bb, colb);
+                gf_mul_cols<M>(d, cold);
+#pragma unroll
+                for (int b = 0; b < NBL; b++) {
+                    if (64 * b > n + 1) continue;
+                    const uint32_t cn = gf_mul_by_cols<M>(colb, c[b]) ^ gf_mul_by_cols<M>(cold, bv[b]);
+                    bv[b] = mle ? c[b] : bv[b];
+                    c[b] = cn;
+                }
+#pragma unroll
+                for (int b = NBL - 1; b >= 0; b--) {
+                    if (64 * b > n + 2) continue;
+                    const uint32_t carry =
+                        b == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)bv[b > 0 ? b - 1 : 0], 63);
+                    bv[b] = shift_up1(bv[b], carry);
+                }
+                bv[NBL - 1] &= keep_last;
+                if (mle) {
+                    L = (uint32_t)n + 1 - L;
+                    bb = d;
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NBL; b++)
+#pragma unroll
+                for (int k = 0; k < M; k++) C[b][k] |= ((c[b] >> k) & 1u) << x;
         }
     }
 
@@ -818,10 +915,14 @@ struct FusedTileKernel {
             for (int x = lane; x < tl.count; x += 64) one_key &= (a.item_key[tl.first + x] == k0);
             one_key = __all(one_key);
         }
-        syndrome(a, tl, one_key, lds, lane);
+        // tiles of a few ciphertexts read only the rows of set bits, ciphertext by ciphertext
+        syndrome(a, tl, one_key && tl.count > SCALAR_BM, lds, lane);
         wave_sync();
         uint32_t C[NBL][M];
-        berlekamp_massey(lds, lane, C);
+        if (tl.count <= SCALAR_BM)
+            berlekamp_massey_scalar(lds, lane, tl.count, C);
+        else
+            berlekamp_massey(lds, lane, C);
         if constexpr (ALG == 0) {
             write_coef_rows(C, lds, lane);
             roots(a, tl, lds, lane, e_aligned);

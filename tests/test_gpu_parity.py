@@ -161,3 +161,26 @@ def test_fused_schedules_agree(ctx, keys, alg, tile, wpg, gpc):
     bad = [i for i in range(len(rows)) if not np.array_equal(got[i], want[i])]
     detail = [(i, int(np.nonzero(got[i] != want[i])[0][0])) for i in bad[:6]]
     assert not bad, f"{len(bad)} rows differ (row, first byte): {detail}"
+
+
+@pytest.mark.parametrize("pid,K", [(348864, 63), (8192128, 63), (8192128, 127), (6688128, 127)])
+def test_late_first_syndrome(ctx, keys, pid, K):
+    """Syndromes zero below index K put the first nonzero discrepancy on a 64-coefficient
+    block boundary (B = x^(K+1)); every path, including the bit-sliced and the small-tile
+    scalar Berlekamp-Massey, matches the oracle."""
+    from _gf2 import late_syndrome_cts
+    sk, kid = keys[pid]
+    ct = late_syndrome_cts(pid, sk, K, 5, seed=K)
+    want = orc.decrypt(pid, sk, ct)
+    _, _, T = orc.key_expand(pid, sk)
+    s = np.stack([orc.synd(pid, T, c) for c in ct])
+    assert np.array_equal(ctx.bm(pid, s), np.stack([orc.bm(pid, x) for x in s]))
+    saved = ctx.schedule(pid)
+    try:
+        for tile in (32, 1):
+            ctx.set_schedule(pid, tile_cts=tile)
+            for mode in (cm.MODE_PLAIN, cm.MODE_FUSED):
+                got = ctx.decrypt([kid] * len(ct), ct, mode=mode)
+                assert np.array_equal(got, want), (tile, mode)
+    finally:
+        ctx.set_schedule(pid, **saved)

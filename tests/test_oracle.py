@@ -302,3 +302,24 @@ def test_reference_driver_bad_katnum():
     for arg in ("0", "-3", "abc", ""):
         r = subprocess.run([exe, arg], capture_output=True)
         assert r.returncode == (-4) & 0xFF
+
+
+@pytest.mark.parametrize("pid,K", [(348864, 63), (8192128, 63), (8192128, 127)])
+def test_late_first_syndrome_vs_reference_build(pid, K):
+    """Ciphertexts whose syndrome is zero below index K (K = 63, 127): the first nonzero
+    discrepancy comes when B = x^(K+1), a coefficient on a 64-lane block boundary of the
+    GPU layouts.  Oracle and reference build agree on them, stage by stage."""
+    if not orc.ref_available(pid):
+        pytest.skip("reference build not present")
+    from _gf2 import late_syndrome_cts
+    sk = orc.shipped_sk(pid)
+    ct = late_syndrome_cts(pid, sk, K, 3, seed=K)
+    ref = orc.Ref(pid)
+    ref.set_key(sk)
+    _, _, T = orc.key_expand(pid, sk)
+    for c in ct:
+        s = orc.synd(pid, T, c)
+        assert not np.any(s[:K]) and s[K] != 0
+        assert np.array_equal(s, ref.synd(c))
+        assert np.array_equal(orc.bm(pid, s), ref.bm(s))
+    assert np.array_equal(orc.decrypt(pid, sk, ct), ref.decrypt(ct))
