@@ -103,13 +103,16 @@ def c2_leg(cm, torch, ctx, steps, threads):
 
 
 def key_pool(cm, ctx, n_keys, seed, threads):
-    """n_keys resident keys split over the 5-set mix: {pid: [(key_id, sk), ...]}."""
+    """n_keys resident keys split over the 5-set mix: {pid: [(key_id, sk), ...]}, and the
+    mean time to admit one key (upload + on-device expansion) per set."""
     per_set = {pid: max(1, int(round(n_keys * w / sum(x[1] for x in MIX)))) for pid, w in MIX}
-    pool = {}
+    pool, load_ms = {}, {}
     for pid, nk in per_set.items():
         sks = cm.gen_keys(pid, seed + pid % 1000, nk, threads=threads)
+        t0 = time.perf_counter()
         pool[pid] = [(ctx.load_key(pid, sks[k]), sks[k]) for k in range(nk)]
-    return pool
+        load_ms[str(pid)] = round((time.perf_counter() - t0) / nk * 1e3, 3)
+    return pool, load_ms
 
 
 def mixed_leg(cm, torch, ctx, pool, n_items, keys_per_set, steps, seed, threads):
@@ -288,7 +291,7 @@ def main():
     ap.add_argument("--serve-keys", type=int, default=1024)
     ap.add_argument("--serve-rates", default="1e4,1e5,1e6")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
-    ap.add_argument("--serve-inflight", type=int, default=4)
+    ap.add_argument("--serve-inflight", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--schedules", default=os.path.join(ROOT, "cm_decrypt__amd", "schedules.json"),
@@ -368,11 +371,12 @@ def main():
         c2 = c2_leg(cm, torch, ctx, max(2, args.plain_steps), threads)
         log(f"6960119 x 4096, one key: plain {c2['plain']['value']:,.0f} dec/s, fused {c2['fused']['value']:,.0f} dec/s")
 
-    serve = mixed = None
+    serve = mixed = key_load_ms = None
     if not (args.no_serve and args.no_mixed):
         t0 = time.perf_counter()
-        pool = key_pool(cm, ctx, args.serve_keys, 17, threads)
-        log(f"key pool: {sum(len(v) for v in pool.values())} resident keys ({time.perf_counter() - t0:.1f}s)")
+        pool, key_load_ms = key_pool(cm, ctx, args.serve_keys, 17, threads)
+        log(f"key pool: {sum(len(v) for v in pool.values())} resident keys ({time.perf_counter() - t0:.1f}s); "
+            f"admission ms per key {key_load_ms}")
         if not args.no_mixed:
             mixed = mixed_leg(cm, torch, ctx, pool, 100000, 64, max(2, args.steps // 2), 23, threads)
             log(f"mixed 5-set batch: {mixed['value']:,.0f} dec/s ({mixed['ms_per_step']:.2f} ms/step) "
@@ -425,6 +429,7 @@ def main():
         "plain_6960119_4096": c2,
         "mixed_batch": mixed,
         "serve": serve,
+        "key_admission_ms": key_load_ms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

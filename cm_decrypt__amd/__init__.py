@@ -131,6 +131,7 @@ _sig("cmd_ctx_memory", _I, _P, _c.POINTER(_U64), _c.POINTER(_U64))
 _sig("cmd_key_load", _I, _P, _I32, _P, _SZ, _c.POINTER(_I32))
 _sig("cmd_key_unload", _I, _P, _I32)
 _sig("cmd_key_device_bytes", _I, _I32, _c.POINTER(_U64))
+_sig("cmd_key_export", _I, _P, _I32, _P, _P, _P, _P, _c.POINTER(_I32))
 _sig("cmd_decrypt", _I, _P, _I32, _SZ, _P, _P, _SZ, _P, _SZ)
 _sig("cmd_decrypt_device", _I, _P, _I32, _SZ, _P, _P, _SZ, _P, _SZ, _P)
 _sig("cmd_batch_create", _I, _P, _I32, _SZ, _P, _SZ, _SZ, _I32, _c.POINTER(_P))
@@ -326,6 +327,19 @@ class Context:
         _check(_lib.cmd_key_load(self._h, param_id, _ptr(sk), sk.size, _c.byref(kid)), "cmd_key_load")
         self.key_param[int(kid.value)] = param_id
         return int(kid.value)
+
+    def export_key(self, key_id: int):
+        """Device arrays of a resident key: (table sb x 2t, L n, lbits m x stride, benes (2m-1) x 2^(m-5))."""
+        p = params(self.key_param[key_id])
+        stride = _I32(0)
+        _check(_lib.cmd_key_export(self._h, key_id, None, None, None, None, _c.byref(stride)), "cmd_key_export")
+        table = np.zeros((p.sb, 2 * p.t), np.uint16)
+        L = np.zeros(p.n, np.uint16)
+        lbits = np.zeros((p.m, stride.value), np.uint32)
+        benes = np.zeros((2 * p.m - 1, 1 << (p.m - 5)), np.uint32)
+        _check(_lib.cmd_key_export(self._h, key_id, _ptr(table), _ptr(L), _ptr(lbits), _ptr(benes), None),
+               "cmd_key_export")
+        return table, L, lbits, benes
 
     def unload_key(self, key_id: int):
         _check(_lib.cmd_key_unload(self._h, key_id), "cmd_key_unload")

@@ -430,7 +430,7 @@ extern "C" int cmd_ctx_destroy(cmd_ctx *ctx)
         if (k.mem) (void)hipFree(k.mem);
     if (ctx->d_keys) (void)hipFree(ctx->d_keys);
     if (ctx->d_fft) (void)hipFree(ctx->d_fft);
-    for (DevBuf *b : {&ctx->ct, &ctx->e, &ctx->s, &ctx->loc, &ctx->idx, &ctx->plan}) dev_free(ctx, *b);
+    for (DevBuf *b : {&ctx->ct, &ctx->e, &ctx->s, &ctx->loc, &ctx->idx, &ctx->plan, &ctx->keystage}) dev_free(ctx, *b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -455,48 +455,63 @@ extern "C" int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes)
     return CMD_OK;
 }
 
+// Offsets of the arrays inside a key's single device allocation.
+struct KeyLayout {
+    size_t off_L, off_lb, off_bn;
+};
+
+static KeyLayout key_layout(int pidx)
+{
+    const ParamDesc &p = kParams[pidx];
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    KeyLayout l;
+    l.off_L = up((size_t)p.sb * 2 * p.t * 2);
+    l.off_lb = l.off_L + up((size_t)p.n * 2);
+    l.off_bn = l.off_lb + up((size_t)p.m * lbits_stride(pidx) * 4);
+    return l;
+}
+
 extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, size_t sk_len, int32_t *key_id)
 {
     if (!ctx || !sk || !key_id) return CMD_ERR_ARG;
     int pidx = param_index(param_id);
     if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
-    ExpandedKey K;
-    int rc = expand_key(pidx, sk, sk_len, K);
-    if (rc) return set_error(rc, "secret key of %zu bytes, %d expected", sk_len, kParams[pidx].sk_bytes);
+    const ParamDesc &p = kParams[pidx];
+    if (sk_len < (size_t)p.sk_bytes)
+        return set_error(CMD_ERR_CRYPTO, "secret key of %zu bytes, %d expected", sk_len, p.sk_bytes);
 
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     CMD_HIP(hipSetDevice(ctx->device));
+    // the Goppa coefficients and control bits go to the device, which expands them
+    const size_t key_bytes = (size_t)2 * p.t + (size_t)p.cond_bytes;
+    int rc = dev_reserve(ctx, ctx->keystage, key_bytes);
+    if (rc) return rc;
     const size_t bytes = key_device_bytes(pidx);
     if (ctx->used + bytes > ctx->budget)
         return set_error(CMD_ERR_BUDGET, "key of %zu bytes exceeds the budget (%llu of %llu used)", bytes,
                          (unsigned long long)ctx->used, (unsigned long long)ctx->budget);
     void *mem = nullptr;
     CMD_HIP(hipMalloc(&mem, bytes));
-    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-    const ParamDesc &p = kParams[pidx];
+    const KeyLayout lay = key_layout(pidx);
     char *base = (char *)mem;
-    size_t off_L = up((size_t)p.sb * 2 * p.t * 2), off_lb = off_L + up((size_t)p.n * 2);
-    size_t off_bn = off_lb + up(K.lbits.size() * 4);
-    hipError_t e1 = hipMemcpy(base, K.table.data(), K.table.size() * 2, hipMemcpyHostToDevice);
-    hipError_t e2 = hipMemcpy(base + off_L, K.L.data(), K.L.size() * 2, hipMemcpyHostToDevice);
-    hipError_t e3 = hipMemcpy(base + off_lb, K.lbits.data(), K.lbits.size() * 4, hipMemcpyHostToDevice);
-    hipError_t e4 = hipMemcpy(base + off_bn, K.benes.data(), K.benes.size() * 4, hipMemcpyHostToDevice);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
-        (void)hipFree(mem);
-        hipError_t bad = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : (e3 != hipSuccess ? e3 : e4));
-        return hip_fail(bad, "key upload");
-    }
     KeySlot slot;
     slot.used = true;
     slot.pidx = pidx;
     slot.mem = mem;
     slot.bytes = bytes;
     slot.dev.table = (const uint16_t *)base;
-    slot.dev.L = (const uint16_t *)(base + off_L);
-    slot.dev.lbits = (const uint32_t *)(base + off_lb);
-    slot.dev.benes = (const uint32_t *)(base + off_bn);
+    slot.dev.L = (const uint16_t *)(base + lay.off_L);
+    slot.dev.lbits = (const uint32_t *)(base + lay.off_lb);
+    slot.dev.benes = (const uint32_t *)(base + lay.off_bn);
     slot.dev.pidx = pidx;
     slot.dev.lstride = lbits_stride(pidx);
+    hipError_t e = hipMemcpyAsync(ctx->keystage.ptr, sk + 40, key_bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_key_expand(pidx, (const uint8_t *)ctx->keystage.ptr, slot.dev, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(mem);
+        return hip_fail(e, "key expansion");
+    }
     int32_t id = -1;
     for (size_t i = 0; i < ctx->keys.size(); i++)
         if (!ctx->keys[i].used) { id = (int32_t)i; break; }
@@ -514,6 +529,26 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
     rc = upload_key_slot(ctx, id);
     if (rc) return rc;
     *key_id = id;
+    return CMD_OK;
+}
+
+extern "C" int cmd_key_export(cmd_ctx *ctx, int32_t key_id, uint16_t *table, uint16_t *L, uint32_t *lbits,
+                              uint32_t *benes, int32_t *lbits_stride_out)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    if (key_id < 0 || (size_t)key_id >= ctx->keys.size() || !ctx->keys[key_id].used)
+        return set_error(CMD_ERR_KEY, "key id %d is not loaded", (int)key_id);
+    CMD_HIP(hipSetDevice(ctx->device));
+    const KeySlot &k = ctx->keys[key_id];
+    const ParamDesc &p = kParams[k.pidx];
+    if (lbits_stride_out) *lbits_stride_out = k.dev.lstride;
+    if (table) CMD_HIP(hipMemcpy(table, k.dev.table, (size_t)p.sb * 2 * p.t * 2, hipMemcpyDeviceToHost));
+    if (L) CMD_HIP(hipMemcpy(L, k.dev.L, (size_t)p.n * 2, hipMemcpyDeviceToHost));
+    if (lbits) CMD_HIP(hipMemcpy(lbits, k.dev.lbits, (size_t)p.m * k.dev.lstride * 4, hipMemcpyDeviceToHost));
+    if (benes)
+        CMD_HIP(hipMemcpy(benes, k.dev.benes, (size_t)(2 * p.m - 1) * ((size_t)1 << (p.m - 5)) * 4,
+                          hipMemcpyDeviceToHost));
     return CMD_OK;
 }
 

@@ -60,6 +60,7 @@ struct cmd_ctx {
 
     // work buffers of cmd_decrypt / cmd_decrypt_device (grow on demand)
     cmd::DevBuf ct, e, s, loc, idx, plan;
+    cmd::DevBuf keystage;  // secret-key bytes for on-device expansion
     uint32_t *d_counter = nullptr;
 
     cmd::FusedSchedule sched[cmd::kNumParams];

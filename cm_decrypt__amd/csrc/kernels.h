@@ -28,6 +28,11 @@ struct FftTab {
 constexpr int kFftVariants = 3;  // (m, l) = (12, 7), (13, 7), (13, 8)
 constexpr int fft_variant(int m, int l) { return m == 12 ? 0 : (l == 7 ? 1 : 2); }
 
+// ---- on-device key expansion ---------------------------------------------------------
+// d_key holds the secret key from byte 40 on: t little-endian Goppa coefficients, then the
+// Benes control bits.  Fills k.L, k.lbits, k.table and k.benes (all of set pidx).
+hipError_t launch_key_expand(int pidx, const uint8_t *d_key, const KeyDev &k, hipStream_t st);
+
 // ---- plain per-stage pipeline (one key per launch group) ----------------------------
 // Items k < count use ciphertext ct + idx[k]*ct_stride and write e + idx[k]*e_stride;
 // idx == nullptr means identity.  s: count x 2t, loc: count x (t+1).

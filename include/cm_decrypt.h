@@ -119,11 +119,19 @@ int cmd_ctx_destroy(cmd_ctx *ctx);
 /* Device memory charged against the budget (keys + work buffers) and the budget. */
 int cmd_ctx_memory(cmd_ctx *ctx, uint64_t *used, uint64_t *budget);
 
-/* Parses, expands and uploads one key; it stays resident until unloaded.
+/* Uploads one key's Goppa coefficients and control bits and expands them on the GPU
+ * (support, syndrome table, bit planes, Benes masks); the key stays resident until unloaded.
  * Replaces initialisation() + compute_inverses() + InitializeC()
  * (GPU_Baseline/param/param8192128/include/decrypt.h:147-173). */
 int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, size_t sk_len, int32_t *key_id);
 int cmd_key_unload(cmd_ctx *ctx, int32_t key_id);
+/* Copies a resident key's device arrays to host memory (any pointer may be NULL):
+ * table sb x 2t uint16 (L_i^j / g(L_i)^2), L n uint16, lbits m x *lbits_stride uint32 (bit k of
+ * L[32w + b] at bit b of word [k][w]), benes (2m-1) x 2^(m-5) uint32 swap masks.  No
+ * counterpart in the reference: a diagnostic that lets callers and tests compare the
+ * on-device key expansion of cmd_key_load with cmd_key_expand_host. */
+int cmd_key_export(cmd_ctx *ctx, int32_t key_id, uint16_t *table, uint16_t *L, uint32_t *lbits, uint32_t *benes,
+                   int32_t *lbits_stride);
 /* Device bytes one resident key of this set occupies. */
 int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes);
 

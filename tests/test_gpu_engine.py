@@ -273,3 +273,49 @@ def test_kernel_timing(ctx, keys):
     finally:
         ctx.set_kernel_timing(False)
     assert n >= 1 and us > 0
+
+
+def _lbits_from_L(m, L, stride):
+    out = np.zeros((m, stride), np.uint32)
+    for k in range(m):
+        bits = ((L >> k) & 1).astype(np.uint8)
+        words = np.packbits(bits, bitorder="little")
+        words = np.concatenate([words, np.zeros((-len(words)) % 4, np.uint8)]).view("<u4")
+        out[k, :len(words)] = words
+    return out
+
+
+def _benes_from_cond(m, cond):
+    count, words = 1 << m, 1 << (m - 5)
+    out = np.zeros((2 * m - 1, words), np.uint32)
+    c = np.arange(count)
+    for k in range(2 * m - 1):
+        s = k if k < m else 2 * m - 2 - k
+        bits = np.unpackbits(cond[k * (count // 16):(k + 1) * (count // 16)], bitorder="little")
+        lower = c[((c >> s) & 1) == 0]
+        j = ((lower >> (s + 1)) << s) | (lower & ((1 << s) - 1))
+        on = lower[bits[j] == 1]
+        np.bitwise_or.at(out[k], on >> 5, (np.uint32(1) << (on & 31).astype(np.uint32)))
+        if s >= 5:
+            np.bitwise_or.at(out[k], (on + (1 << s)) >> 5, (np.uint32(1) << (on & 31).astype(np.uint32)))
+    return out
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_device_key_expansion_matches_host(ctx, pid):
+    """cmd_key_load expands keys on the GPU: support, syndrome table, bit planes and Benes
+    masks equal the host expansion (itself pinned against the reference build) for the shipped
+    key and a generated one."""
+    p = cm.params(pid)
+    for sk in (orc.shipped_sk(pid), cm.gen_keypair(pid, 555)):
+        kid = ctx.load_key(pid, sk)
+        try:
+            table, L, lbits, benes = ctx.export_key(kid)
+        finally:
+            ctx.unload_key(kid)
+        g, L_host, T_host = cm.key_expand_host(pid, sk)
+        assert np.array_equal(L, L_host)
+        assert np.array_equal(table, T_host.reshape(p.sb, 2 * p.t))
+        assert np.array_equal(lbits, _lbits_from_L(p.m, L_host, lbits.shape[1]))
+        cond = np.asarray(sk[40 + 2 * p.t:40 + 2 * p.t + p.cond_bytes], np.uint8)
+        assert np.array_equal(benes, _benes_from_cond(p.m, cond))
