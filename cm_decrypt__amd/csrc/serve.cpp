@@ -9,6 +9,11 @@
 // their completion events; each request's error vector is copied out and its latency
 // (submit -> result available) recorded.
 //
+// Host path: requests live in fixed blocks of a slot pool (never moved, reused through a
+// free list); a request id carries its slot index and a generation number, so lookups need
+// no map and a stale id is rejected.  The lock guards only the queue and the states: the
+// worker copies ciphertexts into a launch and results out of it without holding it.
+//
 // Admission: keys are admitted by the context's HBM budget when loaded; a key with
 // requests in flight cannot be unloaded; the in-flight I/O buffers are allocated up front
 // within io_budget_bytes, and submit() rejects with CMD_ERR_FULL once queue_capacity
@@ -19,8 +24,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <thread>
-#include <unordered_map>
 
 #include "engine.h"
 
@@ -35,15 +40,20 @@ double now_us()
     return std::chrono::duration<double, std::micro>(Clock::now().time_since_epoch()).count();
 }
 
+enum ReqState : uint8_t { kFree = 0, kPending = 1, kInFlight = 2, kDone = 3 };
+
 struct Req {
-    uint64_t id = 0;
+    uint32_t gen = 0;
+    uint8_t state = kFree;
     int32_t key = -1;
-    int pidx = -1;
-    int state = 0;  // 0 pending, 1 in flight, 2 done
-    double t_submit = 0, t_done = 0;
+    int32_t pidx = -1;
+    double t_submit = 0;
     uint8_t ct[kMaxSyndBytes];
     uint8_t e[kMaxEBytes];
 };
+
+constexpr int kBlockBits = 12;  // 4096 requests per pool block
+constexpr uint32_t kBlockSize = 1u << kBlockBits;
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -51,9 +61,9 @@ struct Slot {
     uint8_t *h_ct = nullptr, *h_e = nullptr;  // pinned staging
     char *h_plan = nullptr;                   // pinned plan staging
     DevBuf dev;                               // d_ct | d_e | plan
-    std::vector<Req *> reqs;
-    bool busy = false;
-    double t_launch = 0;
+    std::vector<uint32_t> reqs;               // pool indices of the launch, in item order
+    bool busy = false;                        // reserved by the worker (being launched or in flight)
+    bool launched = false;                    // its completion event is recorded
 };
 
 }  // namespace
@@ -63,20 +73,48 @@ struct cmd_server {
     cmd_server_config cfg{};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<Req *> pending;
-    std::unordered_map<uint64_t, Req *> reqs;
+    // request pool: blocks are allocated on demand and never move
+    std::vector<std::unique_ptr<Req[]>> blocks;
+    std::vector<uint32_t> free_idx;
+    uint32_t pool_size = 0, pool_cap = 0;
+    std::deque<uint32_t> pending;
     std::vector<Slot> slots;
     std::thread worker;
     bool stop = false;
     int worker_rc = CMD_OK;
-    uint64_t next_id = 1;
     // statistics
     uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_items = 0;
     std::vector<double> lat_us;
     double busy_since = -1, busy_us = 0;
     int busy_slots = 0;
-    // per-slot plan sizing
     size_t plan_bytes = 0;
+
+    Req &req(uint32_t idx) { return blocks[idx >> kBlockBits][idx & (kBlockSize - 1)]; }
+
+    // Pool index of a new request, or UINT32_MAX when the pool is exhausted.  mu held.
+    uint32_t alloc()
+    {
+        if (free_idx.empty()) {
+            if (pool_size >= pool_cap) return UINT32_MAX;
+            const uint32_t b = pool_size >> kBlockBits;
+            blocks[b].reset(new (std::nothrow) Req[kBlockSize]);
+            if (!blocks[b]) return UINT32_MAX;
+            for (uint32_t i = kBlockSize; i-- > 0;) free_idx.push_back(pool_size + i);
+            pool_size += kBlockSize;
+        }
+        const uint32_t idx = free_idx.back();
+        free_idx.pop_back();
+        return idx;
+    }
+
+    // The request an id names, or nullptr for an unknown or already taken id.  mu held.
+    Req *lookup(uint64_t id)
+    {
+        const uint32_t idx = (uint32_t)id, gen = (uint32_t)(id >> 32);
+        if (idx >= pool_size) return nullptr;
+        Req &r = req(idx);
+        return (r.state != kFree && r.gen == gen) ? &r : nullptr;
+    }
 };
 
 namespace {
@@ -86,13 +124,14 @@ constexpr size_t kEStride = kMaxEBytes;
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-// Launches the requests in `batch` on slot s.  Called with srv->mu held.
-int launch_slot(cmd_server *srv, Slot &s, std::vector<Req *> &batch)
+// Plans and enqueues the launch of slot s (its reqs already chosen).  Runs without srv->mu:
+// in-flight requests are not touched by other threads.
+int launch_slot(cmd_server *srv, Slot &s)
 {
     cmd_ctx *ctx = srv->ctx;
-    const size_t count = batch.size();
+    const size_t count = s.reqs.size();
     std::vector<int32_t> key_ids(count);
-    for (size_t i = 0; i < count; i++) key_ids[i] = batch[i]->key;
+    for (size_t i = 0; i < count; i++) key_ids[i] = srv->req(s.reqs[i]).key;
     std::vector<FusedTile> tiles;
     std::vector<int32_t> items, item_key;
     int launch_tile = 32;
@@ -113,7 +152,10 @@ int launch_slot(cmd_server *srv, Slot &s, std::vector<Req *> &batch)
         plan_fused(ctx, per_set, count, key_ids.data(), tiles, items, item_key, &launch_tile);
         sched = launch_schedule(per_set, tiles, launch_tile);
     }
-    for (size_t i = 0; i < count; i++) std::memcpy(s.h_ct + i * kCtStride, batch[i]->ct, kParams[batch[i]->pidx].synd_bytes);
+    for (size_t i = 0; i < count; i++) {
+        const Req &r = srv->req(s.reqs[i]);
+        std::memcpy(s.h_ct + i * kCtStride, r.ct, kParams[r.pidx].synd_bytes);
+    }
 
     const size_t mb = (size_t)srv->cfg.max_batch;
     char *base = (char *)s.dev.ptr;
@@ -127,7 +169,11 @@ int launch_slot(cmd_server *srv, Slot &s, std::vector<Req *> &batch)
     std::memcpy(s.h_plan + off_tiles, tiles.data(), tiles.size() * sizeof(FusedTile));
 
     CMD_HIP(hipMemcpyAsync(d_ct, s.h_ct, count * kCtStride, hipMemcpyHostToDevice, s.stream));
-    CMD_HIP(hipMemcpyAsync(d_plan, s.h_plan, off_ctr, hipMemcpyHostToDevice, s.stream));
+    CMD_HIP(hipMemcpyAsync(d_plan, s.h_plan, items.size() * 4, hipMemcpyHostToDevice, s.stream));
+    CMD_HIP(hipMemcpyAsync(d_plan + off_keys, s.h_plan + off_keys, item_key.size() * 4, hipMemcpyHostToDevice,
+                           s.stream));
+    CMD_HIP(hipMemcpyAsync(d_plan + off_tiles, s.h_plan + off_tiles, tiles.size() * sizeof(FusedTile),
+                           hipMemcpyHostToDevice, s.stream));
     FusedArgs a;
     a.tiles = (const FusedTile *)(d_plan + off_tiles);
     a.n_tiles = (int32_t)tiles.size();
@@ -145,50 +191,51 @@ int launch_slot(cmd_server *srv, Slot &s, std::vector<Req *> &batch)
     CMD_HIP(launch_fused(a, sched, ctx->n_cu, s.stream));
     CMD_HIP(hipMemcpyAsync(s.h_e, d_e, count * kEStride, hipMemcpyDeviceToHost, s.stream));
     CMD_HIP(hipEventRecord(s.done, s.stream));
-
-    s.reqs.swap(batch);
-    for (Req *r : s.reqs) r->state = 1;
-    s.busy = true;
-    s.t_launch = now_us();
-    if (srv->busy_slots++ == 0) srv->busy_since = s.t_launch;
-    srv->launches++;
-    srv->launched_items += count;
     return CMD_OK;
 }
 
-void retire_slot(cmd_server *srv, Slot &s)
+// Copies a finished launch's results into its requests (without srv->mu), then marks them
+// done and records statistics (with it).
+void retire_slot(cmd_server *srv, Slot &s, std::unique_lock<std::mutex> &lk, bool ok)
 {
+    lk.unlock();
     const double t = now_us();
+    if (ok)
+        for (size_t i = 0; i < s.reqs.size(); i++) {
+            Req &r = srv->req(s.reqs[i]);
+            std::memcpy(r.e, s.h_e + i * kEStride, kParams[r.pidx].n / 8);
+        }
     {
         std::lock_guard<std::recursive_mutex> g(srv->ctx->lock);
-        for (Req *r : s.reqs) srv->ctx->keys[r->key].inflight--;
+        for (uint32_t idx : s.reqs) srv->ctx->keys[srv->req(idx).key].inflight--;
     }
-    for (size_t i = 0; i < s.reqs.size(); i++) {
-        Req *r = s.reqs[i];
-        std::memcpy(r->e, s.h_e + i * kEStride, kParams[r->pidx].n / 8);
-        r->state = 2;
-        r->t_done = t;
-        srv->lat_us.push_back(t - r->t_submit);
+    lk.lock();
+    for (uint32_t idx : s.reqs) {
+        Req &r = srv->req(idx);
+        r.state = kDone;
+        srv->lat_us.push_back(t - r.t_submit);
         srv->completed++;
     }
     s.reqs.clear();
     s.busy = false;
+    s.launched = false;
     if (--srv->busy_slots == 0) {
         srv->busy_us += t - srv->busy_since;
         srv->busy_since = -1;
     }
 }
 
-// Marks requests finished without a result (after a device error); waiters get worker_rc.
-void fail_requests(cmd_server *srv, std::vector<Req *> &batch)
+// Marks queued requests finished without a result (after a device error); waiters then get
+// worker_rc.  mu held.
+void fail_pending(cmd_server *srv)
 {
     std::lock_guard<std::recursive_mutex> g(srv->ctx->lock);
-    for (Req *r : batch) {
-        srv->ctx->keys[r->key].inflight--;
-        r->state = 2;
-        r->t_done = now_us();
+    for (uint32_t idx : srv->pending) {
+        Req &r = srv->req(idx);
+        srv->ctx->keys[r.key].inflight--;
+        r.state = kDone;
     }
-    batch.clear();
+    srv->pending.clear();
 }
 
 void worker_main(cmd_server *srv)
@@ -198,14 +245,14 @@ void worker_main(cmd_server *srv)
     for (;;) {
         bool progressed = false;
         for (Slot &s : srv->slots) {
-            if (!s.busy) continue;
+            if (!s.busy || !s.launched) continue;
             hipError_t q = hipEventQuery(s.done);
             if (q == hipSuccess) {
-                retire_slot(srv, s);
+                retire_slot(srv, s, lk, true);
                 progressed = true;
             } else if (q != hipErrorNotReady) {
                 srv->worker_rc = hip_fail(q, "server launch");
-                retire_slot(srv, s);  // results are invalid; waiters see the error code
+                retire_slot(srv, s, lk, false);  // results are invalid; waiters see the error code
                 progressed = true;
             }
         }
@@ -215,28 +262,37 @@ void worker_main(cmd_server *srv)
             Slot *free_slot = nullptr;
             for (Slot &s : srv->slots)
                 if (!s.busy) { free_slot = &s; break; }
-            const double waited = now_us() - srv->pending.front()->t_submit;
+            const double waited = now_us() - srv->req(srv->pending.front()).t_submit;
             const bool ready = (int)srv->pending.size() >= srv->cfg.min_batch || waited >= srv->cfg.max_wait_us ||
                                srv->busy_slots == 0 || srv->stop;
             if (free_slot && ready) {
-                std::vector<Req *> batch;
-                while (!srv->pending.empty() && (int)batch.size() < srv->cfg.max_batch) {
-                    batch.push_back(srv->pending.front());
+                Slot &s = *free_slot;
+                while (!srv->pending.empty() && (int)s.reqs.size() < srv->cfg.max_batch) {
+                    const uint32_t idx = srv->pending.front();
                     srv->pending.pop_front();
+                    srv->req(idx).state = kInFlight;
+                    s.reqs.push_back(idx);
                 }
-                int rc = launch_slot(srv, *free_slot, batch);
+                s.busy = true;
+                const double t = now_us();
+                if (srv->busy_slots++ == 0) srv->busy_since = t;
+                srv->launches++;
+                srv->launched_items += s.reqs.size();
+                lk.unlock();
+                const int rc = launch_slot(srv, s);
+                lk.lock();
                 if (rc) {
                     srv->worker_rc = rc;
-                    fail_requests(srv, batch);
+                    retire_slot(srv, s, lk, false);
                     srv->cv_done.notify_all();
+                } else {
+                    s.launched = true;
                 }
                 progressed = true;
             }
         }
         if (srv->worker_rc != CMD_OK && !srv->pending.empty()) {
-            std::vector<Req *> rest(srv->pending.begin(), srv->pending.end());
-            srv->pending.clear();
-            fail_requests(srv, rest);
+            fail_pending(srv);
             srv->cv_done.notify_all();
         }
         if (srv->stop && srv->pending.empty() && srv->busy_slots == 0) break;
@@ -274,7 +330,14 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     srv->ctx = ctx;
     srv->cfg = c;
     srv->plan_bytes = plan;
+    // pending requests plus those of every launch slot, rounded up to whole blocks
+    const uint64_t cap = (uint64_t)c.queue_capacity + (uint64_t)c.max_inflight * (uint64_t)c.max_batch;
+    const uint64_t nblocks = (cap + kBlockSize - 1) / kBlockSize;
+    if (nblocks > (UINT32_MAX >> kBlockBits)) return set_error(CMD_ERR_ARG, "queue capacity too large");
+    srv->blocks.resize(nblocks);
+    srv->pool_cap = (uint32_t)(nblocks << kBlockBits);
     srv->slots.resize(c.max_inflight);
+    for (Slot &s : srv->slots) s.reqs.reserve(mb);
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     CMD_HIP(hipSetDevice(ctx->device));
     int rc = CMD_OK;
@@ -307,61 +370,25 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     return CMD_OK;
 }
 
-extern "C" int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *ct, uint64_t *req)
-{
-    if (!s || !ct || !req) return CMD_ERR_ARG;
-    int pidx;
-    {
-        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-        if (key_id < 0 || (size_t)key_id >= s->ctx->keys.size() || !s->ctx->keys[key_id].used)
-            return set_error(CMD_ERR_KEY, "key id %d is not loaded", (int)key_id);
-        pidx = s->ctx->keys[key_id].pidx;
-    }
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) return set_error(CMD_ERR_ARG, "server is shutting down");
-    if ((int)s->pending.size() >= s->cfg.queue_capacity) {
-        s->rejected++;
-        return set_error(CMD_ERR_FULL, "%d requests pending", (int)s->pending.size());
-    }
-    Req *r = new (std::nothrow) Req();
-    if (!r) return CMD_ERR_NOMEM;
-    r->id = s->next_id++;
-    r->key = key_id;
-    r->pidx = pidx;
-    std::memcpy(r->ct, ct, kParams[pidx].synd_bytes);
-    r->t_submit = now_us();
-    {
-        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-        s->ctx->keys[key_id].inflight++;
-    }
-    s->reqs[r->id] = r;
-    s->pending.push_back(r);
-    s->submitted++;
-    *req = r->id;
-    s->cv_work.notify_one();
-    return CMD_OK;
-}
-
 extern "C" int cmd_server_submit_batch(cmd_server *s, size_t count, const int32_t *key_ids, const uint8_t *ct,
                                        size_t ct_stride, uint64_t *reqs, size_t *accepted)
 {
+    if (accepted) *accepted = 0;
     if (!s || (count && (!key_ids || !ct || !reqs))) return CMD_ERR_ARG;
-    size_t done = 0;
-    int rc = CMD_OK;
+    std::vector<int32_t> pidx(count);
     {
         std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
         for (size_t i = 0; i < count; i++) {
             const int32_t k = key_ids[i];
-            if (k < 0 || (size_t)k >= s->ctx->keys.size() || !s->ctx->keys[k].used) {
-                if (accepted) *accepted = 0;
+            if (k < 0 || (size_t)k >= s->ctx->keys.size() || !s->ctx->keys[k].used)
                 return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)k);
-            }
-            if (ct_stride < (size_t)kParams[s->ctx->keys[k].pidx].synd_bytes) {
-                if (accepted) *accepted = 0;
+            pidx[i] = s->ctx->keys[k].pidx;
+            if (ct_stride < (size_t)kParams[pidx[i]].synd_bytes)
                 return set_error(CMD_ERR_ARG, "ct_stride %zu below the ciphertext size of item %zu", ct_stride, i);
-            }
         }
     }
+    size_t done = 0;
+    int rc = CMD_OK;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) return set_error(CMD_ERR_ARG, "server is shutting down");
@@ -372,41 +399,49 @@ extern "C" int cmd_server_submit_batch(cmd_server *s, size_t count, const int32_
                 rc = set_error(CMD_ERR_FULL, "%d requests pending", (int)s->pending.size());
                 break;
             }
-            Req *r = new (std::nothrow) Req();
-            if (!r) {
-                rc = CMD_ERR_NOMEM;
+            const uint32_t idx = s->alloc();
+            if (idx == UINT32_MAX) {
+                rc = set_error(CMD_ERR_NOMEM, "request pool exhausted");
                 break;
             }
-            r->id = s->next_id++;
-            r->key = key_ids[done];
-            {
-                std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-                r->pidx = s->ctx->keys[r->key].pidx;
-                s->ctx->keys[r->key].inflight++;
-            }
-            std::memcpy(r->ct, ct + done * ct_stride, kParams[r->pidx].synd_bytes);
-            r->t_submit = t;
-            s->reqs[r->id] = r;
-            s->pending.push_back(r);
+            Req &r = s->req(idx);
+            r.gen++;
+            r.state = kPending;
+            r.key = key_ids[done];
+            r.pidx = pidx[done];
+            r.t_submit = t;
+            std::memcpy(r.ct, ct + done * ct_stride, kParams[r.pidx].synd_bytes);
+            s->pending.push_back(idx);
             s->submitted++;
-            reqs[done] = r->id;
+            reqs[done] = ((uint64_t)r.gen << 32) | idx;
+        }
+        if (done) {
+            // held against cmd_key_unload until retired (context lock after the server lock,
+            // the order every path here uses)
+            std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+            for (size_t i = 0; i < done; i++) s->ctx->keys[key_ids[i]].inflight++;
         }
     }
     if (accepted) *accepted = done;
-    s->cv_work.notify_one();
+    if (done) s->cv_work.notify_one();
     return rc;
+}
+
+extern "C" int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *ct, uint64_t *req)
+{
+    if (!s || !ct || !req) return CMD_ERR_ARG;
+    return cmd_server_submit_batch(s, 1, &key_id, ct, kMaxSyndBytes, req, nullptr);
 }
 
 static int take_result(cmd_server *s, uint64_t id, uint8_t *e_out)
 {
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)id);
-    Req *r = it->second;
-    if (r->state != 2) return CMD_ERR_PENDING;
+    Req *r = s->lookup(id);
+    if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+    if (r->state != kDone) return CMD_ERR_PENDING;
     if (s->worker_rc) return s->worker_rc;
     if (e_out) std::memcpy(e_out, r->e, kParams[r->pidx].n / 8);
-    s->reqs.erase(it);
-    delete r;
+    r->state = kFree;
+    s->free_idx.push_back((uint32_t)id);
     return CMD_OK;
 }
 
@@ -441,9 +476,9 @@ extern "C" int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t
     std::unique_lock<std::mutex> lk(s->mu);
     const auto deadline = Clock::now() + std::chrono::microseconds((long long)(timeout_us < 0 ? 0 : timeout_us));
     for (size_t i = 0; i < count; i++) {
-        auto it = s->reqs.find(reqs[i]);
-        if (it == s->reqs.end()) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)reqs[i]);
-        if ((size_t)kParams[it->second->pidx].n / 8 > e_stride)
+        const Req *r = s->lookup(reqs[i]);
+        if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)reqs[i]);
+        if ((size_t)kParams[r->pidx].n / 8 > e_stride)
             return set_error(CMD_ERR_ARG, "e_stride %zu below the error vector of request %zu", e_stride, i);
     }
     for (size_t i = 0; i < count; i++) {
@@ -522,7 +557,6 @@ extern "C" int cmd_server_destroy(cmd_server *s)
         if (sl.h_plan) (void)hipHostFree(sl.h_plan);
         dev_free(s->ctx, sl.dev);
     }
-    for (auto &kv : s->reqs) delete kv.second;  // every request was retired by the worker
-    delete s;
+    delete s;  // every request was retired by the worker; the pool blocks go with the server
     return CMD_OK;
 }
