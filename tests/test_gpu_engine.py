@@ -319,3 +319,30 @@ def test_device_key_expansion_matches_host(ctx, pid):
         assert np.array_equal(lbits, _lbits_from_L(p.m, L_host, lbits.shape[1]))
         cond = np.asarray(sk[40 + 2 * p.t:40 + 2 * p.t + p.cond_bytes], np.uint8)
         assert np.array_equal(benes, _benes_from_cond(p.m, cond))
+
+
+CLI = os.path.join(orc.ROOT, "cm_decrypt__amd", "cmdecrypt")
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+def test_cli_drop_in_matches_reference_errorstream(tmp_path, mode):
+    """The CLI invoked as decrypt_348864 in the reference's directory layout reads
+    ct/sk_348864.bin and writes errorstream0_348864.bin byte-identical to the one the reference
+    driver wrote for the same files (golden fixture)."""
+    import subprocess
+    gold = np.load(os.path.join(orc.GOLDEN, "golden_c1_348864.npz"))
+    vec = tmp_path / "kem" / "test_vectors" / "Cipher_Sk"
+    vec.mkdir(parents=True)
+    gold["sk"].tofile(vec / "sk_348864.bin")
+    gold["ct"].tofile(vec / "ct_348864.bin")
+    work = tmp_path / "CPU" / "param" / "param348864"
+    work.mkdir(parents=True)
+    (tmp_path / "CPU" / "results" / "output").mkdir(parents=True)
+    exe = work / "decrypt_348864"
+    os.symlink(CLI, exe)
+    r = subprocess.run([str(exe), str(len(gold["ct"])), "--mode", mode], cwd=work, capture_output=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr.decode()
+    out = (tmp_path / "CPU" / "results" / "output" / "errorstream0_348864.bin").read_bytes()
+    assert out == gold["stream"].tobytes()
