@@ -130,16 +130,44 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
-// Folds planes 2M-2 .. M of an unreduced product back into planes 0 .. M-1.
+// Which high planes of an unreduced product land in plane k after reduction: x^s mod the
+// field polynomial for s = M .. 2M-2, as a table computed at compile time.
+template <int M>
+struct FoldMap {
+    int count[M];
+    int src[M][M];
+};
+
+template <int M>
+constexpr FoldMap<M> make_fold_map()
+{
+    FoldMap<M> f{};
+    constexpr uint32_t poly = (1u << M) | Modulus<M>::low;
+    for (int s = M; s <= 2 * M - 2; s++) {
+        uint32_t v = 1u << s;
+        for (int b = 2 * M - 2; b >= M; b--)
+            if ((v >> b) & 1u) v ^= poly << (b - M);
+        for (int k = 0; k < M; k++)
+            if ((v >> k) & 1u) f.src[k][f.count[k]++] = s;
+    }
+    return f;
+}
+
+// Folds planes 2M-2 .. M of an unreduced product back into planes 0 .. M-1: each low plane
+// takes its high sources two at a time with one three-input XOR (30 instructions for
+// GF(2^13), 14 for GF(2^12), against 48 and 22 for the plane-by-plane fold).
 template <int M>
 __device__ __forceinline__ void bs_fold_planes(uint32_t p[2 * M - 1])
 {
-    constexpr uint32_t low = Modulus<M>::low;
+    constexpr FoldMap<M> F = make_fold_map<M>();
 #pragma unroll
-    for (int s = 2 * M - 2; s >= M; s--)
+    for (int k = 0; k < M; k++) {
+        uint32_t acc = p[k];
 #pragma unroll
-        for (int b2 = 0; b2 < 5; b2++)
-            if ((low >> b2) & 1u) p[s - M + b2] ^= p[s];
+        for (int i = 0; i + 1 < F.count[k]; i += 2) acc = xor3(acc, p[F.src[k][i]], p[F.src[k][i + 1]]);
+        if (F.count[k] & 1) acc ^= p[F.src[k][F.count[k] - 1]];
+        p[k] = acc;
+    }
 }
 
 // out = a * b for 32 element pairs (bit lanes).  out may alias a or b.

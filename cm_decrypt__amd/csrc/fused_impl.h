@@ -26,6 +26,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <utility>
 
 #include "dev_gf.h"
@@ -65,6 +66,12 @@ __device__ __forceinline__ uint32_t wave_xor_all(uint32_t v)
 __device__ __forceinline__ uint32_t shift_up1(uint32_t v, uint32_t carry)
 {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)carry, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
+
+// Lane l receives v from lane l-1; lane 0 receives 0 (one DPP move).
+__device__ __forceinline__ uint32_t shift_up1_zero(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, true);
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -381,7 +388,7 @@ struct FusedTileKernel {
                 bs_mul_add2<M>(bb, C[b], d, B[b], cn);
 #pragma unroll
                 for (int k = 0; k < M; k++) {
-                    B[b][k] = (C[b][k] & mle) | (B[b][k] & ~mle);
+                    B[b][k] = __builtin_amdgcn_bitop3_b32(mle, C[b][k], B[b][k], 0xCA);  // mle ? C : B
                     C[b][k] = cn[k];
                 }
             }
@@ -392,9 +399,12 @@ struct FusedTileKernel {
                 if (64 * b > n + 2) continue;
 #pragma unroll
                 for (int k = 0; k < M; k++) {
-                    const uint32_t carry =
-                        b == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)B[b > 0 ? b - 1 : 0][k], 63);
-                    B[b][k] = shift_up1(B[b][k], carry);
+                    if (b == 0) {
+                        B[b][k] = shift_up1_zero(B[b][k]);
+                    } else {
+                        const uint32_t carry = (uint32_t)__builtin_amdgcn_readlane((int)B[b > 0 ? b - 1 : 0][k], 63);
+                        B[b][k] = shift_up1(B[b][k], carry);
+                    }
                 }
             }
 #pragma unroll
@@ -968,14 +978,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 template <int ALG, int OCC>
 static hipError_t launch_fused_variant(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
 {
-    static bool configured = false;
+    // the dynamic-LDS limit of the kernel is raised once per process (any thread may launch)
+    static std::once_flag once;
+    static hipError_t set_rc = hipSuccess;
+    std::call_once(once, [] {
+        set_rc = hipFuncSetAttribute((const void *)k_fused<ALG, OCC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     4 * kWaveLds * 4);
+    });
+    if (set_rc != hipSuccess) return set_rc;
     const size_t lds_bytes = (size_t)s.waves_per_group * kWaveLds * 4;
-    if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_fused<ALG, OCC>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 4 * kWaveLds * 4);
-        if (e != hipSuccess) return e;
-        configured = true;
-    }
     int groups = n_cu * s.groups_per_cu;
     const int max_groups = (a.n_tiles + s.waves_per_group - 1) / s.waves_per_group;
     if (groups > max_groups) groups = max_groups;
