@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <thread>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -337,38 +338,129 @@ extern "C" int cmd_key_expand_host(int32_t param_id, const uint8_t *sk, size_t s
     return CMD_OK;
 }
 
+// ---- errorstream text (write_error_positions, CPU/param/param6960119/src/decrypt.c:55-66) ----
+// " %d" for every set bit in ascending order, then "\n", per vector.  Vectors are formatted in
+// parallel chunks: each chunk first measures its text, then writes at its offset.
+
+namespace {
+
+struct DigitTable {
+    char text[8192][4];  // decimal digits of 0 .. 8191, left aligned
+    uint8_t len[8192];
+    DigitTable()
+    {
+        for (int i = 0; i < 8192; i++) {
+            char tmp[8];
+            int n = 0, v = i;
+            do { tmp[n++] = (char)('0' + v % 10); v /= 10; } while (v);
+            len[i] = (uint8_t)n;
+            for (int k = 0; k < n; k++) text[i][k] = tmp[n - 1 - k];
+        }
+    }
+};
+
+const DigitTable &digits()
+{
+    static const DigitTable t;
+    return t;
+}
+
+// Bytes of one vector's line.
+size_t line_bytes(const uint8_t *v, int nbytes)
+{
+    const DigitTable &d = digits();
+    size_t pos = 1;  // "\n"
+    int byte = 0;
+    for (; byte + 8 <= nbytes; byte += 8) {
+        uint64_t w;
+        std::memcpy(&w, v + byte, 8);
+        while (w) {
+            pos += 1 + d.len[byte * 8 + __builtin_ctzll(w)];
+            w &= w - 1;
+        }
+    }
+    for (; byte < nbytes; byte++)
+        for (uint32_t b = v[byte]; b; b &= b - 1) pos += 1 + d.len[byte * 8 + __builtin_ctz(b)];
+    return pos;
+}
+
+// Writes one vector's line at out; returns its length.
+size_t line_write(const uint8_t *v, int nbytes, char *out)
+{
+    const DigitTable &d = digits();
+    char *p = out;
+    auto put = [&](int i) {
+        *p++ = ' ';
+        std::memcpy(p, d.text[i], 4);  // the table row has room for 4 digits
+        p += d.len[i];
+    };
+    int byte = 0;
+    for (; byte + 8 <= nbytes; byte += 8) {
+        uint64_t w;
+        std::memcpy(&w, v + byte, 8);
+        while (w) {
+            put(byte * 8 + __builtin_ctzll(w));
+            w &= w - 1;
+        }
+    }
+    for (; byte < nbytes; byte++)
+        for (uint32_t b = v[byte]; b; b &= b - 1) put(byte * 8 + __builtin_ctz(b));
+    *p++ = '\n';
+    return (size_t)(p - out);
+}
+
+// Runs fn(first, last) over [0, count) in up to hardware_concurrency chunks.
+template <typename F>
+void parallel_chunks(size_t count, F fn)
+{
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::max(1u, std::min(nt, 16u));
+    if (count < 2048) nt = 1;
+    std::vector<std::thread> pool;
+    const size_t per = (count + nt - 1) / nt;
+    for (unsigned t = 1; t < nt; t++) {
+        const size_t a = t * per, b = std::min(count, a + per);
+        if (a < b) pool.emplace_back(fn, a, b);
+    }
+    fn(0, std::min(count, per));
+    for (auto &th : pool) th.join();
+}
+
+}  // namespace
+
 extern "C" int cmd_errorstream_format(int32_t param_id, const uint8_t *e, size_t e_stride, size_t count, char *buf,
                                       size_t cap, size_t *written)
 {
     int pidx = param_index(param_id);
     if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
-    const int n = kParams[pidx].n;
-    if ((!e && count) || e_stride < (size_t)n / 8) return CMD_ERR_ARG;
-    size_t pos = 0;
-    char tmp[8];
-    for (size_t k = 0; k < count; k++) {
-        const uint8_t *v = e + k * e_stride;
-        for (int byte = 0; byte < n / 8; byte++) {
-            uint32_t bits = v[byte];
-            while (bits) {
-                int i = byte * 8 + __builtin_ctz(bits);
-                bits &= bits - 1;
-                int len = 0;
-                do { tmp[len++] = (char)('0' + i % 10); i /= 10; } while (i);
-                if (pos < cap) buf[pos] = ' ';
-                pos++;
-                while (len) {
-                    char c = tmp[--len];
-                    if (pos < cap) buf[pos] = c;
-                    pos++;
-                }
+    const int nbytes = kParams[pidx].n / 8;
+    if ((!e && count) || e_stride < (size_t)nbytes) return CMD_ERR_ARG;
+    std::vector<size_t> off(count + 1, 0);
+    parallel_chunks(count, [&](size_t a, size_t b) {
+        for (size_t k = a; k < b; k++) off[k + 1] = line_bytes(e + k * e_stride, nbytes);
+    });
+    for (size_t k = 0; k < count; k++) off[k + 1] += off[k];
+    const size_t total = off[count];
+    if (written) *written = total;
+    if (!buf || cap == 0) return CMD_OK;
+    if (total > cap) return set_error(CMD_ERR_ARG, "buffer of %zu bytes, %zu needed", cap, total);
+    // the last 3 bytes of a line may be overwritten by the 4-byte digit copy of the next
+    // number; lines are written in order within a chunk, and each chunk's last line is
+    // written with exact-size copies into a bounce buffer
+    parallel_chunks(count, [&](size_t a, size_t b) {
+        for (size_t k = a; k < b; k++) {
+            const uint8_t *v = e + k * e_stride;
+            const size_t len = off[k + 1] - off[k];
+            if (k + 1 < b && off[k + 1] + 3 < total) {
+                line_write(v, nbytes, buf + off[k]);
+            } else {
+                std::vector<char> tmp(len + 4);
+                line_write(v, nbytes, tmp.data());
+                std::memcpy(buf + off[k], tmp.data(), len);
             }
         }
-        if (pos < cap) buf[pos] = '\n';
-        pos++;
-    }
-    if (written) *written = pos;
-    return pos <= cap || cap == 0 ? CMD_OK : set_error(CMD_ERR_ARG, "buffer of %zu bytes, %zu needed", cap, pos);
+    });
+    return CMD_OK;
 }
 
 extern "C" int cmd_errorstream_write(int32_t param_id, const char *path, const uint8_t *e, size_t e_stride,
@@ -378,7 +470,7 @@ extern "C" int cmd_errorstream_write(int32_t param_id, const char *path, const u
     int rc = cmd_errorstream_format(param_id, e, e_stride, count, nullptr, 0, &need);
     if (rc) return rc;
     std::vector<char> buf(need);
-    rc = cmd_errorstream_format(param_id, e, e_stride, count, buf.data(), buf.size(), &need);
+    if (need) rc = cmd_errorstream_format(param_id, e, e_stride, count, buf.data(), buf.size(), &need);
     if (rc) return rc;
     FILE *f = std::fopen(path, append ? "ab" : "wb");
     if (!f) return set_error(CMD_ERR_FILE, "cannot open %s", path);

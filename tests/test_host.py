@@ -165,3 +165,23 @@ def test_no_device_is_reported(monkeypatch):
     with pytest.raises(cm.CmdError) as ei:
         cm.Context(0)
     assert ei.value.code == cm.CMD_ERR_NODEV
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_errorstream_parallel_formatter_vs_oracle(pid):
+    """Large batches are formatted in parallel chunks; every line equals the oracle's
+    (write_error_positions), for dense random bytes, empty and weight-t vectors."""
+    p = cm.params(pid)
+    rng = np.random.default_rng(pid % 1000)
+    e = rng.integers(0, 256, (5000, p.e_bytes), dtype=np.uint8)
+    e[::7] = 0
+    for i in range(1, 5000, 5):
+        e[i] = 0
+        pos = rng.choice(p.n, p.t, replace=False)
+        np.bitwise_or.at(e[i], pos >> 3, (1 << (pos & 7)).astype(np.uint8))
+    text = cm.errorstream(pid, e)
+    lines = text.split(b"\n")
+    assert len(lines) == len(e) + 1 and lines[-1] == b""
+    for i in range(0, len(e), 13):
+        assert lines[i] + b"\n" == orc.errorstream_line(pid, e[i]), i
+    assert cm.errorstream(pid, e[:100]) == b"".join(orc.errorstream_line(pid, x) for x in e[:100])
