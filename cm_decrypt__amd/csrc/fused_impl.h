@@ -76,6 +76,11 @@ __device__ __forceinline__ uint32_t shift_up1_zero(uint32_t v)
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Key tables and masks live in device global memory; saying so lets loads use a scalar base
+// plus a 32-bit lane offset instead of a 64-bit address per lane (flat).
+using gu32 = const __attribute__((address_space(1))) uint32_t;
+__device__ __forceinline__ gu32 *gptr(const uint32_t *p) { return (gu32 *)p; }
+
 // The same value, hidden from the optimiser.  Values derived from the lane id inside a loop
 // would otherwise be hoisted to the kernel entry, where hundreds of them no longer fit in
 // registers and are spilled; deriving them from an opaque copy keeps them where they are used.
@@ -180,39 +185,57 @@ struct FusedTileKernel {
     static constexpr int PAIRS = T;                      // table dwords per row (2t uint16)
     static constexpr int R = (PAIRS + 63) / 64;
     static constexpr int WORDS = (Pm::synd_bytes + 3) / 4;
+    static_assert(Pm::sb % 16 == 0, "row groups come in pairs");
     static constexpr int CH = TILE;                      // ciphertexts per syndrome chunk
     static constexpr int CST = (T + 2) & ~1;             // LDS uint16 stride of a coefficient row
     static_assert(TILE * CST * 2 <= kWaveLds * 4, "coefficient rows must fit the wave's LDS");
     static_assert(CH == TILE && CH % 16 == 0, "one syndrome chunk covers the tile");
 
-    // Eight table rows (bits 8G .. 8G+7 of a ciphertext word) into the accumulators of the CH
-    // ciphertexts of a single-key chunk.  All eight rows' loads are issued before the first
-    // use; the per-ciphertext bit masks are wave-uniform and made together, which lets the
-    // compiler keep them in SGPRs (scalar s_bfe) so each term costs one v_bitop3.
-    template <int G>
-    static __device__ __forceinline__ void syndrome_rows(const uint32_t *wrows, const uint32_t (&cw)[CH],
-                                                         uint32_t (&acc)[CH][R], int lane)
+    // A row group is the eight table rows of one ciphertext byte (sb is a multiple of 8).
+    static constexpr int NGROUP = Pm::sb / 8;
+
+    static __device__ __forceinline__ void load_group(gu32 *rows, int g, int lane, uint32_t (&v)[8][R])
     {
-        uint32_t v[8][R];
+        gu32 *base = rows + (size_t)(8 * g) * PAIRS;
 #pragma unroll
         for (int q = 0; q < 8; q++)
 #pragma unroll
-            for (int r = 0; r < R; r++)
-                v[q][r] = (lane + 64 * r < PAIRS) ? wrows[(size_t)(8 * G + q) * PAIRS + lane + 64 * r] : 0u;
+            for (int r = 0; r < R; r++) v[q][r] = (lane + 64 * r < PAIRS) ? base[(size_t)q * PAIRS + lane + 64 * r] : 0u;
+    }
+
+    // The eight rows of group g into the accumulators of the CH ciphertexts of a single-key
+    // chunk: row 8g+q is added where bit BIT0+q of the ciphertext's word is set (BIT0 = 8 (g mod
+    // 4), a constant so the masks stay scalar).  The masks are wave-uniform and made sixteen at
+    // a time in SGPRs (s_bfe), so each term costs one v_bitop3.
+    template <int BIT0>
+    static __device__ __forceinline__ void add_group(const uint32_t (&v)[8][R], const uint32_t (&cw)[CH],
+                                                     uint32_t (&acc)[CH][R])
+    {
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            // sixteen masks at a time bounds the SGPRs they hold
 #pragma unroll
             for (int x0 = 0; x0 < CH; x0 += 16) {
                 uint32_t msk[16];
 #pragma unroll
-                for (int x = 0; x < 16; x++) msk[x] = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x0 + x], 8 * G + q, 1);
+                for (int x = 0; x < 16; x++) msk[x] = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x0 + x], BIT0 + q, 1);
 #pragma unroll
                 for (int x = 0; x < 16; x++)
 #pragma unroll
                     for (int r = 0; r < R; r++) acc[x0 + x][r] = and_xor(acc[x0 + x][r], v[q][r], msk[x]);
             }
         }
+    }
+
+    // Bytes 4w .. 4w+3 of a ciphertext (those that exist) as a little-endian word.
+    static __device__ __forceinline__ uint32_t ct_word(const uint8_t *c, int w)
+    {
+        uint32_t v = 0;
+        if (c) {
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if (4 * w + b < Pm::synd_bytes) v |= (uint32_t)c[4 * w + b] << (8 * b);
+        }
+        return v;
     }
 
     // ---- stage 1: syndromes into LDS bit planes ---------------------------------------------
@@ -235,39 +258,28 @@ struct FusedTileKernel {
                 for (int r = 0; r < R; r++) acc[x][r] = 0;
 
             if (one_key) {
+                // Row groups are software-pipelined through two register buffers: the loads of
+                // the next group are in flight while the current one is added, hiding the L2
+                // latency.  One iteration is one ciphertext word (four groups); 6960119's last
+                // word has two.
                 const KeyDev *kd = &a.keys[a.item_key[tl.first + x0]];
-                const uint32_t *rows = reinterpret_cast<const uint32_t *>(kd->table);
+                gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
+                uint32_t va[8][R], vb[8][R], cw[CH];
+                load_group(rows, 0, lane, va);
                 for (int w = 0; w < WORDS; w++) {
-                    uint32_t mine = 0;
-                    if (my_ct) {
-#pragma unroll
-                        for (int b = 0; b < 4; b++)
-                            if (4 * w + b < Pm::synd_bytes) mine |= (uint32_t)my_ct[4 * w + b] << (8 * b);
-                    }
-                    uint32_t cw[CH];
+                    const uint32_t mine = ct_word(my_ct, w);
 #pragma unroll
                     for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
-                    const uint32_t *wrows = rows + (size_t)(w * 32) * PAIRS;
-                    if (w * 32 + 32 <= Pm::sb) {
-                        syndrome_rows<0>(wrows, cw, acc, lane);
-                        syndrome_rows<1>(wrows, cw, acc, lane);
-                        syndrome_rows<2>(wrows, cw, acc, lane);
-                        syndrome_rows<3>(wrows, cw, acc, lane);
-                    } else {
-                        // partial last word (6960119: 16 bits)
-                        for (int bit = 0; bit < Pm::sb - w * 32; bit++) {
-                            uint32_t v[R];
-#pragma unroll
-                            for (int r = 0; r < R; r++)
-                                v[r] = (lane + 64 * r < PAIRS) ? wrows[(size_t)bit * PAIRS + lane + 64 * r] : 0u;
-#pragma unroll
-                            for (int x = 0; x < CH; x++) {
-                                const uint32_t msk = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x], bit, 1);
-#pragma unroll
-                                for (int r = 0; r < R; r++) acc[x][r] = and_xor(acc[x][r], v[r], msk);
-                            }
-                        }
-                    }
+                    const int g = 4 * w;
+                    if (g + 1 < NGROUP) load_group(rows, g + 1, lane, vb);
+                    add_group<0>(va, cw, acc);
+                    if (g + 2 < NGROUP) load_group(rows, g + 2, lane, va);
+                    if (g + 1 < NGROUP) add_group<8>(vb, cw, acc);
+                    if (g + 2 >= NGROUP) break;
+                    load_group(rows, g + 3, lane, vb);
+                    add_group<16>(va, cw, acc);
+                    if (g + 4 < NGROUP) load_group(rows, g + 4, lane, va);
+                    add_group<24>(vb, cw, acc);
                 }
             } else {
                 // one key per ciphertext: each row feeds one accumulator.  The ciphertext is
