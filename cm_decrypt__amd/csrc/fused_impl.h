@@ -290,7 +290,7 @@ struct FusedTileKernel {
                 for (int x = 0; x < CH; x++) {
                     if (x0 + x >= tl.count) continue;
                     const KeyDev *kd = &a.keys[a.item_key[tl.first + x0 + x]];
-                    const uint32_t *rows = reinterpret_cast<const uint32_t *>(kd->table);
+                    gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
                     const uint8_t *ctp = a.ct + (size_t)a.items[tl.first + x0 + x] * a.ct_stride;
                     for (int w = 0; w < WORDS; w++) {
                         uint32_t cwv = 0;
@@ -298,7 +298,7 @@ struct FusedTileKernel {
                         for (int b = 0; b < 4; b++)
                             if (4 * w + b < Pm::synd_bytes) cwv |= (uint32_t)ctp[4 * w + b] << (8 * b);
                         uint32_t bits = uni(cwv);
-                        const uint32_t *wrows = rows + (size_t)(w * 32) * PAIRS;
+                        gu32 *wrows = rows + (size_t)(w * 32) * PAIRS;
                         while (bits) {
                             uint32_t v[GRP][R];
 #pragma unroll
@@ -553,7 +553,7 @@ struct FusedTileKernel {
         for (int x = 0; x < tl.count; x++) {
             const int item = a.items[tl.first + x];
             const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
-            const uint32_t *lb = kd->lbits;
+            gu32 *lb = gptr(kd->lbits);
             const int ls = kd->lstride;
             uint8_t *eo = a.e + (size_t)item * a.e_stride;
             const uint16_t *cr = coef + x * CST;
@@ -833,11 +833,11 @@ struct FusedTileKernel {
     }
 
     template <int K>
-    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], const uint32_t *masks, int lane,
+    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], gu32 *masks, int lane,
                                                        uint32_t wl_fin)
     {
         constexpr int S = K < M ? K : 2 * M - 2 - K;
-        const uint32_t *mrow = masks + K * NW + opaque((int)wl_fin);
+        gu32 *mrow = masks + K * NW + opaque((int)wl_fin);
         uint32_t mw[WPL];
 #pragma unroll
         for (int r = 0; r < WPL; r++) mw[r] = mrow[reg_part(SCHED.fin, r, 64)];
@@ -874,7 +874,7 @@ struct FusedTileKernel {
     }
 
     template <int... Ks>
-    static __device__ __forceinline__ void benes_all(uint32_t (&z)[WPL], const uint32_t *masks, int lane,
+    static __device__ __forceinline__ void benes_all(uint32_t (&z)[WPL], gu32 *masks, int lane,
                                                      uint32_t wl_fin, std::integer_sequence<int, Ks...>)
     {
         (benes_layer<Ks>(z, masks, lane, wl_fin), ...);
@@ -913,7 +913,7 @@ struct FusedTileKernel {
                 for (int k = 0; k < M; k++) nz |= W[r][k];
                 z[r] = ~nz;
             }
-            benes_all(z, kd->benes, lane, wl_fin, std::make_integer_sequence<int, 2 * M - 1>{});
+            benes_all(z, gptr(kd->benes), lane, wl_fin, std::make_integer_sequence<int, 2 * M - 1>{});
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
                 const int g = (int)final_word(wl_fin, r);
