@@ -31,6 +31,8 @@ LIB_PATH = os.path.join(_HERE, "libcmdecrypt.so")
 PARAM_IDS = (348864, 460896, 6688128, 6960119, 8192128)
 MODE_PLAIN = 0
 MODE_FUSED = 1
+MODE_KEM = 2           # rows out are 32-byte session keys (crypto_kem_dec)
+KEM_KEY_BYTES = 32
 
 CMD_OK = 0
 CMD_ERR_FILE = -1
@@ -138,6 +140,8 @@ _sig("cmd_batch_create", _I, _P, _I32, _SZ, _P, _SZ, _SZ, _I32, _c.POINTER(_P))
 _sig("cmd_batch_run", _I, _P, _P, _P, _P)
 _sig("cmd_batch_destroy", _I, _P)
 _sig("cmd_stage_synd", _I, _P, _I32, _SZ, _P, _SZ, _P)
+_sig("cmd_stage_synd_full", _I, _P, _I32, _SZ, _P, _SZ, _P)
+_sig("cmd_kem_dec", _I, _P, _SZ, _P, _P, _SZ, _P, _SZ)
 _sig("cmd_stage_bm", _I, _P, _I32, _SZ, _P, _P)
 _sig("cmd_stage_root", _I, _P, _I32, _SZ, _P, _P, _SZ)
 _sig("cmd_schedule_get", _I, _P, _I32, _c.POINTER(Schedule))
@@ -365,6 +369,18 @@ class Context:
                "cmd_decrypt")
         return e
 
+    def kem_dec(self, key_ids: Sequence[int], ct: np.ndarray, ct_stride: Optional[int] = None) -> np.ndarray:
+        """Session keys (count x 32 bytes) of crypto_kem_dec for ciphertext rows `ct`."""
+        keys = np.ascontiguousarray(key_ids, dtype=np.int32)
+        ct = np.ascontiguousarray(ct, dtype=np.uint8)
+        count = keys.size
+        if ct_stride is None:
+            ct_stride = ct.shape[1] if ct.ndim == 2 else ct.size // max(count, 1)
+        out = np.zeros((count, KEM_KEY_BYTES), np.uint8)
+        _check(_lib.cmd_kem_dec(self._h, count, _ptr(keys), _ptr(ct), ct_stride, _ptr(out), KEM_KEY_BYTES),
+               "cmd_kem_dec")
+        return out
+
     def decrypt_device(self, key_ids, d_ct: int, ct_stride: int, d_e: int, e_stride: int,
                        mode: int = MODE_FUSED, stream: Optional[int] = None):
         keys = np.ascontiguousarray(key_ids, dtype=np.int32)
@@ -377,6 +393,15 @@ class Context:
         ct = np.ascontiguousarray(ct, dtype=np.uint8).reshape(-1, p.ct_bytes)
         s = np.zeros((ct.shape[0], 2 * p.t), np.uint16)
         _check(_lib.cmd_stage_synd(self._h, key_id, ct.shape[0], _ptr(ct), p.ct_bytes, _ptr(s)), "cmd_stage_synd")
+        return s
+
+    def synd_full(self, key_id: int, r: np.ndarray) -> np.ndarray:
+        """s_j(r) = sum_{i<n} r_i L_i^j / g(L_i)^2, j < 2t, for rows r of n/8 bytes."""
+        p = params(self.key_param[key_id])
+        r = np.ascontiguousarray(r, dtype=np.uint8).reshape(-1, p.e_bytes)
+        s = np.zeros((r.shape[0], 2 * p.t), np.uint16)
+        _check(_lib.cmd_stage_synd_full(self._h, key_id, r.shape[0], _ptr(r), p.e_bytes, _ptr(s)),
+               "cmd_stage_synd_full")
         return s
 
     def bm(self, param_id: int, s: np.ndarray) -> np.ndarray:

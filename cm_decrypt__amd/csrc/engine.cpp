@@ -76,16 +76,27 @@ FusedSchedule default_schedule(int pidx)
     return s;
 }
 
-static size_t key_device_bytes(int pidx)
+// Offsets of the arrays inside a key's single device allocation, and its size.
+struct KeyLayout {
+    size_t off_L, off_lb, off_bn, off_y, off_s, bytes;
+};
+
+static KeyLayout key_layout(int pidx)
 {
     const ParamDesc &p = kParams[pidx];
-    size_t table = (size_t)p.sb * 2 * p.t * 2;
-    size_t L = (size_t)p.n * 2;
-    size_t lb = (size_t)p.m * lbits_stride(pidx) * 4;
-    size_t bn = (size_t)(2 * p.m - 1) * ((size_t)1 << (p.m - 5)) * 4;
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-    return up(table) + up(L) + up(lb) + up(bn);
+    const size_t words = (size_t)1 << (p.m - 5);
+    KeyLayout l;
+    l.off_L = up((size_t)p.sb * 2 * p.t * 2);
+    l.off_lb = l.off_L + up((size_t)p.n * 2);
+    l.off_bn = l.off_lb + up((size_t)p.m * lbits_stride(pidx) * 4);
+    l.off_y = l.off_bn + up((size_t)(2 * p.m - 1) * words * 4);
+    l.off_s = l.off_y + up((size_t)p.m * words * 4);
+    l.bytes = l.off_s + up((size_t)p.n / 8);
+    return l;
 }
+
+static size_t key_device_bytes(int pidx) { return key_layout(pidx).bytes; }
 
 static int upload_key_slot(cmd_ctx *ctx, int32_t id)
 {
@@ -106,6 +117,9 @@ void batch_release(cmd_batch *b)
     dev_free(b->ctx, b->d_plan);
     dev_free(b->ctx, b->d_s);
     dev_free(b->ctx, b->d_loc);
+    dev_free(b->ctx, b->d_kem_e);
+    dev_free(b->ctx, b->d_ok);
+    dev_free(b->ctx, b->d_msg);
 }
 
 int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, const int32_t *key_ids,
@@ -167,9 +181,10 @@ int batch_plan(cmd_batch *b, const int32_t *key_ids)
         max_ct = std::max(max_ct, (size_t)p.synd_bytes);
         max_e = std::max(max_e, (size_t)p.n / 8);
     }
-    if (b->ct_stride < max_ct || b->e_stride < max_e)
+    const size_t max_out = b->mode == CMD_MODE_KEM ? (size_t)CMD_KEM_KEY_BYTES : max_e;
+    if (b->ct_stride < max_ct || b->e_stride < max_out)
         return set_error(CMD_ERR_ARG, "strides (%zu, %zu) below item sizes (%zu, %zu)", b->ct_stride, b->e_stride,
-                         max_ct, max_e);
+                         max_ct, max_out);
 
     // order items by (set, key), keeping input order inside a key
     std::vector<int32_t> order(count);
@@ -197,7 +212,7 @@ int batch_plan(cmd_batch *b, const int32_t *key_ids)
         i = j;
     }
 
-    if (b->mode == CMD_MODE_FUSED) {
+    if (b->mode == CMD_MODE_FUSED || b->mode == CMD_MODE_KEM) {
         int rc = plan_fused(ctx, b->sched, count, key_ids, b->tiles, b->items, b->item_key, &b->launch_tile);
         if (rc) return rc;
     } else {
@@ -228,6 +243,14 @@ int batch_plan(cmd_batch *b, const int32_t *key_ids)
     if (b->mode == CMD_MODE_PLAIN && b->max_group > 0) {
         rc = dev_alloc(ctx, b->d_s, (size_t)b->max_group * 2 * b->max_t * 2);
         if (!rc) rc = dev_alloc(ctx, b->d_loc, (size_t)b->max_group * (b->max_t + 1) * 2);
+        if (rc) return rc;
+    }
+    if (b->mode == CMD_MODE_KEM && count > 0) {
+        // decoded error vectors, accept flags and SHAKE256 inputs stay on the device
+        b->kem_e_stride = (max_e + 3) / 4 * 4;
+        rc = dev_alloc(ctx, b->d_kem_e, count * b->kem_e_stride);
+        if (!rc) rc = dev_alloc(ctx, b->d_ok, count * 4);
+        if (!rc) rc = dev_alloc(ctx, b->d_msg, count * (size_t)kKemMsgStride);
         if (rc) return rc;
     }
     return CMD_OK;
@@ -263,9 +286,46 @@ int batch_enqueue(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
     a.e_stride = b->e_stride;
     a.tile_counter = b->d_counter;
     a.fft = ctx->d_fft;
+    const bool kem = b->mode == CMD_MODE_KEM;
+    if (kem) {
+        a.e = (uint8_t *)b->d_kem_e.ptr;
+        a.e_stride = b->kem_e_stride;
+    }
     const FusedSchedule s = launch_schedule(b->sched, b->tiles, b->launch_tile);
     CMD_HIP(launch_zero_u32(b->d_counter, 1, st));
     CMD_HIP(launch_fused(a, s, ctx->n_cu, st));
+    if (!kem) return CMD_OK;
+
+    DecapArgs d;
+    d.tiles = b->d_tiles;
+    d.n_tiles = (int32_t)b->tiles.size();
+    d.pad0 = 0;
+    d.items = b->d_items;
+    d.item_key = b->d_item_key;
+    d.keys = ctx->d_keys;
+    d.ct = ct;
+    d.e = (const uint8_t *)b->d_kem_e.ptr;
+    d.ct_stride = b->ct_stride;
+    d.e_stride = b->kem_e_stride;
+    d.ok = (uint32_t *)b->d_ok.ptr;
+    d.s_out = nullptr;
+    d.fft = ctx->d_fft;
+    CMD_HIP(launch_decap_check(d, st));
+    KemHashArgs h;
+    h.count = (int32_t)b->items.size();
+    h.pad0 = 0;
+    h.items = b->d_items;
+    h.item_key = b->d_item_key;
+    h.keys = ctx->d_keys;
+    h.ct = ct;
+    h.e = (const uint8_t *)b->d_kem_e.ptr;
+    h.ct_stride = b->ct_stride;
+    h.e_stride = b->kem_e_stride;
+    h.ok = (const uint32_t *)b->d_ok.ptr;
+    h.msg = (uint8_t *)b->d_msg.ptr;
+    h.key_out = e;
+    h.key_stride = b->e_stride;
+    CMD_HIP(launch_kem_hash(h, st));
     return CMD_OK;
 }
 
@@ -547,22 +607,6 @@ extern "C" int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes)
     return CMD_OK;
 }
 
-// Offsets of the arrays inside a key's single device allocation.
-struct KeyLayout {
-    size_t off_L, off_lb, off_bn;
-};
-
-static KeyLayout key_layout(int pidx)
-{
-    const ParamDesc &p = kParams[pidx];
-    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-    KeyLayout l;
-    l.off_L = up((size_t)p.sb * 2 * p.t * 2);
-    l.off_lb = l.off_L + up((size_t)p.n * 2);
-    l.off_bn = l.off_lb + up((size_t)p.m * lbits_stride(pidx) * 4);
-    return l;
-}
-
 extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, size_t sk_len, int32_t *key_id)
 {
     if (!ctx || !sk || !key_id) return CMD_ERR_ARG;
@@ -574,8 +618,8 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
 
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     CMD_HIP(hipSetDevice(ctx->device));
-    // the Goppa coefficients and control bits go to the device, which expands them
-    const size_t key_bytes = (size_t)2 * p.t + (size_t)p.cond_bytes;
+    // the Goppa coefficients, control bits and s go to the device, which expands them
+    const size_t key_bytes = (size_t)2 * p.t + (size_t)p.cond_bytes + (size_t)p.n / 8;
     int rc = dev_reserve(ctx, ctx->keystage, key_bytes);
     if (rc) return rc;
     const size_t bytes = key_device_bytes(pidx);
@@ -595,10 +639,15 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
     slot.dev.L = (const uint16_t *)(base + lay.off_L);
     slot.dev.lbits = (const uint32_t *)(base + lay.off_lb);
     slot.dev.benes = (const uint32_t *)(base + lay.off_bn);
+    slot.dev.ypl = (const uint32_t *)(base + lay.off_y);
+    slot.dev.s = (const uint8_t *)(base + lay.off_s);
     slot.dev.pidx = pidx;
     slot.dev.lstride = lbits_stride(pidx);
     hipError_t e = hipMemcpyAsync(ctx->keystage.ptr, sk + 40, key_bytes, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_key_expand(pidx, (const uint8_t *)ctx->keystage.ptr, slot.dev, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + lay.off_s, (const char *)ctx->keystage.ptr + 2 * p.t + p.cond_bytes, (size_t)p.n / 8,
+                           hipMemcpyDeviceToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         (void)hipFree(mem);
@@ -664,7 +713,8 @@ extern "C" int cmd_key_unload(cmd_ctx *ctx, int32_t key_id)
 static int make_batch(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids, size_t ct_stride,
                       size_t e_stride, int32_t use_graph, cmd_batch **out)
 {
-    if (mode != CMD_MODE_PLAIN && mode != CMD_MODE_FUSED) return set_error(CMD_ERR_ARG, "mode %d", (int)mode);
+    if (mode != CMD_MODE_PLAIN && mode != CMD_MODE_FUSED && mode != CMD_MODE_KEM)
+        return set_error(CMD_ERR_ARG, "mode %d", (int)mode);
     if (count && !key_ids) return CMD_ERR_ARG;
     if (count > (size_t)INT32_MAX) return set_error(CMD_ERR_ARG, "count %zu too large", count);
     std::unique_ptr<cmd_batch> b(new cmd_batch());
@@ -801,6 +851,12 @@ extern "C" int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32
     return CMD_OK;
 }
 
+extern "C" int cmd_kem_dec(cmd_ctx *ctx, size_t count, const int32_t *key_ids, const uint8_t *ct, size_t ct_stride,
+                           uint8_t *key_out, size_t key_stride)
+{
+    return cmd_decrypt(ctx, CMD_MODE_KEM, count, key_ids, ct, ct_stride, key_out, key_stride);
+}
+
 // ---- per-stage entry points ----------------------------------------------------------------
 
 static int check_key(cmd_ctx *ctx, int32_t key_id)
@@ -871,6 +927,64 @@ extern "C" int cmd_stage_root(cmd_ctx *ctx, int32_t key_id, size_t count, const 
     CMD_HIP(launch_plain_chien(k.pidx, k.dev, (int)count, nullptr, (const uint16_t *)ctx->loc.ptr,
                                (uint8_t *)ctx->e.ptr, e_stride, ctx->stream));
     CMD_HIP(hipMemcpyAsync(e_out, ctx->e.ptr, count * e_stride, hipMemcpyDeviceToHost, ctx->stream));
+    CMD_HIP(hipStreamSynchronize(ctx->stream));
+    return CMD_OK;
+}
+
+// Syndrome of n-bit words over the whole support (the decapsulation check's s_cmp): one key,
+// tiles of 32 words in input order through the transposed-FFT check kernel.
+extern "C" int cmd_stage_synd_full(cmd_ctx *ctx, int32_t key_id, size_t count, const uint8_t *r, size_t r_stride,
+                                   uint16_t *s_out)
+{
+    if (!ctx || (count && (!r || !s_out))) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    int rc = check_key(ctx, key_id);
+    if (rc) return rc;
+    if (!count) return CMD_OK;
+    if (count > (size_t)INT32_MAX) return CMD_ERR_ARG;
+    CMD_HIP(hipSetDevice(ctx->device));
+    const KeySlot &k = ctx->keys[key_id];
+    const ParamDesc &p = kParams[k.pidx];
+    if (r_stride < (size_t)p.n / 8) return CMD_ERR_ARG;
+    std::vector<FusedTile> tiles;
+    std::vector<int32_t> items(count), item_key(count, key_id);
+    std::iota(items.begin(), items.end(), 0);
+    for (size_t i = 0; i < count; i += 32) {
+        FusedTile t;
+        t.pidx = k.pidx;
+        t.first = (int32_t)i;
+        t.count = (int32_t)std::min<size_t>(32, count - i);
+        t.pad = 0;
+        tiles.push_back(t);
+    }
+    const size_t s_bytes = count * 2 * p.t * 2;
+    const size_t off_key = (count * 4 + 255) / 256 * 256, off_tiles = off_key + off_key;
+    rc = dev_reserve(ctx, ctx->e, count * r_stride);
+    if (!rc) rc = dev_reserve(ctx, ctx->s, s_bytes);
+    if (!rc) rc = dev_reserve(ctx, ctx->plan, off_tiles + tiles.size() * sizeof(FusedTile));
+    if (rc) return rc;
+    char *plan = (char *)ctx->plan.ptr;
+    CMD_HIP(hipMemcpyAsync(ctx->e.ptr, r, count * r_stride, hipMemcpyHostToDevice, ctx->stream));
+    CMD_HIP(hipMemcpyAsync(plan, items.data(), count * 4, hipMemcpyHostToDevice, ctx->stream));
+    CMD_HIP(hipMemcpyAsync(plan + off_key, item_key.data(), count * 4, hipMemcpyHostToDevice, ctx->stream));
+    CMD_HIP(hipMemcpyAsync(plan + off_tiles, tiles.data(), tiles.size() * sizeof(FusedTile), hipMemcpyHostToDevice,
+                           ctx->stream));
+    DecapArgs d;
+    d.tiles = (const FusedTile *)(plan + off_tiles);
+    d.n_tiles = (int32_t)tiles.size();
+    d.pad0 = 0;
+    d.items = (const int32_t *)plan;
+    d.item_key = (const int32_t *)(plan + off_key);
+    d.keys = ctx->d_keys;
+    d.ct = nullptr;
+    d.e = (const uint8_t *)ctx->e.ptr;
+    d.ct_stride = 0;
+    d.e_stride = r_stride;
+    d.ok = nullptr;
+    d.s_out = (uint16_t *)ctx->s.ptr;
+    d.fft = ctx->d_fft;
+    CMD_HIP(launch_decap_check(d, ctx->stream));
+    CMD_HIP(hipMemcpyAsync(s_out, ctx->s.ptr, s_bytes, hipMemcpyDeviceToHost, ctx->stream));
     CMD_HIP(hipStreamSynchronize(ctx->stream));
     return CMD_OK;
 }

@@ -95,6 +95,8 @@ struct cmd_batch {
     cmd::FusedTile *d_tiles = nullptr;
     int32_t *d_items = nullptr, *d_item_key = nullptr;
     cmd::DevBuf d_s, d_loc;   // plain intermediates
+    cmd::DevBuf d_kem_e, d_ok, d_msg;  // KEM: decoded error vectors, accept flags, SHAKE256 inputs
+    size_t kem_e_stride = 0;
     uint32_t *d_counter = nullptr;
     cmd::FusedSchedule sched[cmd::kNumParams];
 

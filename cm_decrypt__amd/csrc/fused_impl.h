@@ -176,6 +176,330 @@ __device__ __forceinline__ uint32_t bitrev_bits(uint32_t v, int bits)
     return __builtin_bitreverse32(v) >> (32 - bits);
 }
 
+// Index layout of the bit-sliced additive FFT over the 2^M field elements (32 evaluation
+// points per word) with 2^LOG leaves, and its linear steps together with their transposes.
+// The forward direction evaluates a polynomial of 2^LOG coefficients at every point; the
+// transposed direction maps values at the points to the power sums sum_c v_c a_c^p, p < 2^LOG,
+// by the same steps in reverse order, each replaced by its transpose (a conditional swap and a
+// multiplication by a constant are their own transposes; "target ^= source" becomes
+// "source ^= target").
+template <int M, int LOG>
+struct FftPlan {
+    static constexpr int NP = (1 << LOG) / 64;         // positions per lane in the coefficient phase
+    static constexpr int NW = 1 << (M - 5);            // 32-point words per ciphertext
+    static constexpr int WPL = NW / 64;                // words per lane in the butterfly phase
+    static constexpr int WPLEAF = 1 << (M - 5 - LOG);  // words covered by one leaf
+
+    // ---- coefficient phase (bit-sliced across the tile, lane = coefficient position) ---------
+
+    template <int D, int LANEX, int PBX>
+    static __device__ __forceinline__ void radix_pass(uint32_t (&F)[NP][M], int lane, int s, int sel)
+    {
+        // XOR into targets (bits s-1, s-2 of the node-local index == sel) the values at p ^ (LANEX + 64 PBX)
+#pragma unroll
+        for (int pb = 0; pb < NP; pb++) {
+            const int p = lane + 64 * pb;
+            const int i = p >> D;
+            const uint32_t hit = (((i >> (s - 2)) & 3) == sel) ? ~0u : 0u;
+            if ((pb ^ PBX) >= NP) continue;
+#pragma unroll
+            for (int k = 0; k < M; k++) F[pb][k] ^= xor_lanes_any<LANEX>(F[pb ^ PBX][k]) & hit;
+        }
+    }
+
+    // Radix sub-levels S .. 2 of depth D: pass 1 targets node-local bits (S-1, S-2) = (1,0) from
+    // distance q, pass 2 targets (0,1) from distance 3q; q = 2^(S-2) node-local = 2^(S-2+D) positions.
+    template <int D, int S>
+    static __device__ __forceinline__ void radix_levels(uint32_t (&F)[NP][M], int lane)
+    {
+        if constexpr (S >= 2) {
+            constexpr int Q = 1 << (S - 2 + D);
+            radix_pass<D, (Q & 63), (Q >> 6)>(F, lane, S, 2);
+            radix_pass<D, ((3 * Q) & 63), ((3 * Q) >> 6)>(F, lane, S, 1);
+            radix_levels<D, S - 1>(F, lane);
+        }
+    }
+
+    template <int D>
+    static __device__ __forceinline__ void coeff_depth(uint32_t (&F)[NP][M], int lane, const FftTab *tab)
+    {
+        if constexpr (D > 0) {
+#pragma unroll
+            for (int pb = 0; pb < NP; pb++) bs_mul_scalar<M>(F[pb], tab->tw[D][lane + 64 * pb]);
+        }
+        radix_levels<D, LOG - D>(F, lane);
+    }
+
+    // Transpose of radix_pass: the values at the targets (node-local bits (s-1, s-2) == sel) are
+    // XORed into the positions p ^ X they were read from.
+    template <int D, int LANEX, int PBX>
+    static __device__ __forceinline__ void radix_pass_tr(uint32_t (&F)[NP][M], int lane, int s, int sel)
+    {
+        constexpr int X = LANEX + 64 * PBX;
+#pragma unroll
+        for (int pb = 0; pb < NP; pb++) {
+            const int p = lane + 64 * pb;
+            const int i = (p ^ X) >> D;
+            const uint32_t hit = (((i >> (s - 2)) & 3) == sel) ? ~0u : 0u;
+            if ((pb ^ PBX) >= NP) continue;
+#pragma unroll
+            for (int k = 0; k < M; k++) F[pb][k] ^= xor_lanes_any<LANEX>(F[pb ^ PBX][k]) & hit;
+        }
+    }
+
+    // Transpose of radix_levels<D, S>: sub-levels 2 .. S, each pass 2 before pass 1.
+    template <int D, int S>
+    static __device__ __forceinline__ void radix_levels_tr(uint32_t (&F)[NP][M], int lane)
+    {
+        if constexpr (S >= 2) {
+            radix_levels_tr<D, S - 1>(F, lane);
+            constexpr int Q = 1 << (S - 2 + D);
+            radix_pass_tr<D, ((3 * Q) & 63), ((3 * Q) >> 6)>(F, lane, S, 1);
+            radix_pass_tr<D, (Q & 63), (Q >> 6)>(F, lane, S, 2);
+        }
+    }
+
+    template <int D>
+    static __device__ __forceinline__ void coeff_depth_tr(uint32_t (&F)[NP][M], int lane, const FftTab *tab)
+    {
+        radix_levels_tr<D, LOG - D>(F, lane);
+        if constexpr (D > 0) {
+#pragma unroll
+            for (int pb = 0; pb < NP; pb++) bs_mul_scalar<M>(F[pb], tab->tw[D][lane + 64 * pb]);
+        }
+    }
+
+
+    // ---- phase B data layout --------------------------------------------------------------
+    // Word w (32 evaluation points) of a ciphertext lives in lane l, register r.  Its index bits
+    // start as w = l + 64 r.  Butterfly level J pairs words that differ in index bit J; when that
+    // bit is a lane bit, it is first exchanged with a register bit (a 2 x 2 transpose between
+    // lane pairs), so every butterfly runs inside one lane on half the words: no product is
+    // computed twice.  The exchange schedule is fixed at compile time: when bit J is in a lane,
+    // it replaces the register bit whose index is needed furthest in the future (or never).
+    static constexpr int RBITS = WPL == 4 ? 2 : 1;      // register bits of the word index
+    static constexpr int NLEV = LOG;                    // butterfly levels J = JMIN .. JMIN+LOG-1
+    static constexpr int JMIN = M - 5 - LOG;
+    struct Layout {
+        int posr[2];   // word-index bit held by register bit q
+        int posl[6];   // word-index bit held by lane bit i
+    };
+    struct Schedule {
+        int swap_lane[8];   // lane bit exchanged with register bit swap_q[i] before level i (-1: none)
+        int swap_q[8];
+        int bq[8];          // register bit holding bit J at level i
+        Layout at[8];       // layout during level i
+        Layout fin;         // layout after the last level
+    };
+    static constexpr Schedule make_schedule()
+    {
+        Schedule sc{};
+        Layout cur{};
+        for (int q = 0; q < 2; q++) cur.posr[q] = q < RBITS ? 6 + q : -1;
+        for (int i = 0; i < 6; i++) cur.posl[i] = i;
+        for (int i = 0; i < NLEV; i++) {
+            const int J = JMIN + i;
+            int q_hold = -1;
+            for (int q = 0; q < RBITS; q++)
+                if (cur.posr[q] == J) q_hold = q;
+            sc.swap_lane[i] = -1;
+            sc.swap_q[i] = -1;
+            if (q_hold < 0) {
+                int lb = 0;
+                for (int b = 0; b < 6; b++)
+                    if (cur.posl[b] == J) lb = b;
+                int best = 0, key = -1;
+                for (int q = 0; q < RBITS; q++) {
+                    const int k = cur.posr[q] < J ? 1000 : cur.posr[q];
+                    if (k > key) { key = k; best = q; }
+                }
+                sc.swap_lane[i] = lb;
+                sc.swap_q[i] = best;
+                cur.posl[lb] = cur.posr[best];
+                cur.posr[best] = J;
+                q_hold = best;
+            }
+            sc.bq[i] = q_hold;
+            sc.at[i] = cur;
+        }
+        sc.fin = cur;
+        return sc;
+    }
+    static constexpr Schedule SCHED = make_schedule();
+
+    // Index bits of lane l's words that come from the lane: bits of l placed where the layout
+    // says, restricted to word-index bits below LIMIT.
+    static __device__ __forceinline__ uint32_t lane_part(const Layout &lay, int lane, int limit)
+    {
+        uint32_t w = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+            if (lay.posl[i] < limit) w |= (uint32_t)((lane >> i) & 1) << lay.posl[i];
+        return w;
+    }
+    static constexpr uint32_t reg_part(const Layout &lay, int r, int limit)
+    {
+        uint32_t w = 0;
+        for (int q = 0; q < RBITS; q++)
+            if (lay.posr[q] < limit) w |= (uint32_t)((r >> q) & 1) << lay.posr[q];
+        return w;
+    }
+
+    // Exchanges lane bit LB with register bit Q: lane l keeps the words whose register bit Q
+    // equals its lane bit LB and trades the others with lane l ^ 2^LB.
+    template <int LB, int Q>
+    static __device__ __forceinline__ void swap_bits(uint32_t (&W)[WPL][M], int lane)
+    {
+        const bool upper = (opaque(lane) >> LB) & 1;
+#pragma unroll
+        for (int r0 = 0; r0 < WPL; r0++) {
+            if ((r0 >> Q) & 1) continue;
+            const int r1 = r0 | (1 << Q);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const uint32_t send = upper ? W[r0][k] : W[r1][k];
+                const uint32_t recv = xor_lanes<(1 << LB)>(send);
+                W[r0][k] = upper ? recv : W[r0][k];
+                W[r1][k] = upper ? W[r1][k] : recv;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // Butterfly level Dl (word-index bit J = M - 6 - Dl): w0 = u + G v, w1 = w0 + v, where
+    // G = lowpat (by bit position) + hi (by the low J bits of the lower word's index).
+    template <int Dl>
+    static __device__ __forceinline__ void butterfly(uint32_t (&W)[WPL][M], int lane, const FftTab *tab)
+    {
+        constexpr int J = M - 6 - Dl;
+        constexpr int LI = J - JMIN;
+        constexpr int SL = SCHED.swap_lane[LI], SQ = SCHED.swap_q[LI];
+        if constexpr (SL >= 0) swap_bits<(SL >= 0 ? SL : 0), (SQ >= 0 ? SQ : 0)>(W, lane);
+        constexpr int RB = 1 << SCHED.bq[LI];
+        constexpr Layout LAY = SCHED.at[LI];
+        // Reload this level's constants here: hoisted out of the per-ciphertext loop, the
+        // multipliers of all levels would not fit in registers and would go to scratch.
+        asm volatile("" ::: "memory");
+        uint32_t lowp[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
+        const uint32_t wl = lane_part(LAY, opaque(lane), J);
+#pragma unroll
+        for (int r = 0; r < WPL; r++) {
+            if (r & RB) continue;
+            const uint32_t hv = tab->hi[Dl][wl | reg_part(LAY, r, J)];
+            uint32_t G[M], t[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) G[k] = lowp[k] ^ bitmask(hv, k);
+            bs_mul<M>(G, W[r | RB], t);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                W[r][k] ^= t[k];
+                W[r | RB][k] ^= W[r][k];
+            }
+            // one word pair at a time: interleaving the products would multiply the live
+            // registers
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // Full word index of lane l's register r in the final layout.
+    static __device__ __forceinline__ uint32_t final_word(uint32_t wl_fin, int r)
+    {
+        return wl_fin | reg_part(SCHED.fin, r, 64);
+    }
+
+    template <int K>
+    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], gu32 *masks, int lane,
+                                                       uint32_t wl_fin)
+    {
+        constexpr int S = K < M ? K : 2 * M - 2 - K;
+        gu32 *mrow = masks + K * NW + opaque((int)wl_fin);
+        uint32_t mw[WPL];
+#pragma unroll
+        for (int r = 0; r < WPL; r++) mw[r] = mrow[reg_part(SCHED.fin, r, 64)];
+        if constexpr (S < 5) {
+            constexpr int SH = 1 << S;
+#pragma unroll
+            for (int r = 0; r < WPL; r++) {
+                const uint32_t t = ((z[r] >> SH) ^ z[r]) & mw[r];
+                z[r] ^= t ^ (t << SH);
+            }
+        } else {
+            constexpr int JB = S - 5;  // word-index bit
+            constexpr int QB = SCHED.fin.posr[0] == JB ? 0 : (RBITS > 1 && SCHED.fin.posr[1] == JB ? 1 : -1);
+            if constexpr (QB >= 0) {
+                constexpr int RB = 1 << QB;
+#pragma unroll
+                for (int r = 0; r < WPL; r++) {
+                    if (r & RB) continue;
+                    const uint32_t d = (z[r] ^ z[r | RB]) & mw[r];
+                    z[r] ^= d;
+                    z[r | RB] ^= d;
+                }
+            } else {
+                constexpr int LBB = SCHED.fin.posl[0] == JB ? 0 : SCHED.fin.posl[1] == JB ? 1
+                                  : SCHED.fin.posl[2] == JB ? 2 : SCHED.fin.posl[3] == JB ? 3
+                                  : SCHED.fin.posl[4] == JB ? 4 : 5;
+#pragma unroll
+                for (int r = 0; r < WPL; r++) {
+                    const uint32_t other = xor_lanes<(1 << LBB)>(z[r]);
+                    z[r] ^= (z[r] ^ other) & mw[r];
+                }
+            }
+        }
+    }
+
+    template <int... Ks>
+    static __device__ __forceinline__ void benes_all(uint32_t (&z)[WPL], gu32 *masks, int lane,
+                                                     uint32_t wl_fin, std::integer_sequence<int, Ks...>)
+    {
+        (benes_layer<Ks>(z, masks, lane, wl_fin), ...);
+    }
+
+    // Transpose of butterfly<Dl>: (a, b) -> (a + b, G (a + b) + b), then the lane/register
+    // exchange that preceded the forward level, which undoes itself.
+    template <int Dl>
+    static __device__ __forceinline__ void butterfly_tr(uint32_t (&W)[WPL][M], int lane, const FftTab *tab)
+    {
+        constexpr int J = M - 6 - Dl;
+        constexpr int LI = J - JMIN;
+        constexpr int SL = SCHED.swap_lane[LI], SQ = SCHED.swap_q[LI];
+        constexpr int RB = 1 << SCHED.bq[LI];
+        constexpr Layout LAY = SCHED.at[LI];
+        asm volatile("" ::: "memory");
+        uint32_t lowp[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
+        const uint32_t wl = lane_part(LAY, opaque(lane), J);
+#pragma unroll
+        for (int r = 0; r < WPL; r++) {
+            if (r & RB) continue;
+            const uint32_t hv = tab->hi[Dl][wl | reg_part(LAY, r, J)];
+            uint32_t G[M], t[M];
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                G[k] = lowp[k] ^ bitmask(hv, k);
+                W[r][k] ^= W[r | RB][k];
+            }
+            bs_mul<M>(G, W[r], t);
+#pragma unroll
+            for (int k = 0; k < M; k++) W[r | RB][k] ^= t[k];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if constexpr (SL >= 0) swap_bits<(SL >= 0 ? SL : 0), (SQ >= 0 ? SQ : 0)>(W, lane);
+    }
+
+    // Benes layers last to first: the transpose (and inverse) of benes_all.
+    template <int... Ks>
+    static __device__ __forceinline__ void benes_all_rev(uint32_t (&z)[WPL], gu32 *masks, int lane,
+                                                         uint32_t wl_fin, std::integer_sequence<int, Ks...>)
+    {
+        (benes_layer<2 * M - 2 - Ks>(z, masks, lane, wl_fin), ...);
+    }
+};
+
+
 template <int IDX, int ALG>
 struct FusedTileKernel {
     static constexpr int TILE = 32;  // bit-sliced width (ciphertext slots per wavefront tile)
@@ -602,49 +926,9 @@ struct FusedTileKernel {
     // The key's own Benes network (the one support_gen applies to bitrev(i), root_13.c:181-211)
     // then moves the root indicator of index c to support position i, giving e directly.
     static constexpr int LOG = (T + 1 > 128) ? 8 : 7;  // 2^LOG >= t + 1 coefficients
-    static constexpr int NP = (1 << LOG) / 64;         // positions per lane in phase A
-    static constexpr int NW = 1 << (M - 5);            // 32-point words per ciphertext
-    static constexpr int WPL = NW / 64;                // words per lane in phase B
-    static constexpr int WPLEAF = 1 << (M - 5 - LOG);  // words covered by one leaf
+    using FP = FftPlan<M, LOG>;
+    static constexpr int NP = FP::NP, NW = FP::NW, WPL = FP::WPL, WPLEAF = FP::WPLEAF;
     static_assert(TILE * (1 << LOG) * 2 <= kWaveLds * 4, "leaf rows must fit the wave's LDS");
-
-    template <int D, int LANEX, int PBX>
-    static __device__ __forceinline__ void radix_pass(uint32_t (&F)[NP][M], int lane, int s, int sel)
-    {
-        // XOR into targets (bits s-1, s-2 of the node-local index == sel) the values at p ^ (LANEX + 64 PBX)
-#pragma unroll
-        for (int pb = 0; pb < NP; pb++) {
-            const int p = lane + 64 * pb;
-            const int i = p >> D;
-            const uint32_t hit = (((i >> (s - 2)) & 3) == sel) ? ~0u : 0u;
-            if ((pb ^ PBX) >= NP) continue;
-#pragma unroll
-            for (int k = 0; k < M; k++) F[pb][k] ^= xor_lanes_any<LANEX>(F[pb ^ PBX][k]) & hit;
-        }
-    }
-
-    // Radix sub-levels S .. 2 of depth D: pass 1 targets node-local bits (S-1, S-2) = (1,0) from
-    // distance q, pass 2 targets (0,1) from distance 3q; q = 2^(S-2) node-local = 2^(S-2+D) positions.
-    template <int D, int S>
-    static __device__ __forceinline__ void radix_levels(uint32_t (&F)[NP][M], int lane)
-    {
-        if constexpr (S >= 2) {
-            constexpr int Q = 1 << (S - 2 + D);
-            radix_pass<D, (Q & 63), (Q >> 6)>(F, lane, S, 2);
-            radix_pass<D, ((3 * Q) & 63), ((3 * Q) >> 6)>(F, lane, S, 1);
-            radix_levels<D, S - 1>(F, lane);
-        }
-    }
-
-    template <int D>
-    static __device__ __forceinline__ void coeff_depth(uint32_t (&F)[NP][M], int lane, const FftTab *tab)
-    {
-        if constexpr (D > 0) {
-#pragma unroll
-            for (int pb = 0; pb < NP; pb++) bs_mul_scalar<M>(F[pb], tab->tw[D][lane + 64 * pb]);
-        }
-        radix_levels<D, LOG - D>(F, lane);
-    }
 
     static __device__ __forceinline__ void fft_coeff_phase(const uint32_t (&C)[NBL][M], uint32_t *lds, int lane,
                                                            const FftTab *tab)
@@ -669,14 +953,14 @@ struct FusedTileKernel {
             for (int k = 0; k < M; k++) F[pb][k] = live ? lds[row * kRowWords + k] : 0u;
         }
         wave_sync();
-        coeff_depth<0>(F, lane, tab);
-        coeff_depth<1>(F, lane, tab);
-        coeff_depth<2>(F, lane, tab);
-        coeff_depth<3>(F, lane, tab);
-        coeff_depth<4>(F, lane, tab);
-        coeff_depth<5>(F, lane, tab);
-        coeff_depth<6>(F, lane, tab);
-        if constexpr (LOG == 8) coeff_depth<7>(F, lane, tab);
+        FP::template coeff_depth<0>(F, lane, tab);
+        FP::template coeff_depth<1>(F, lane, tab);
+        FP::template coeff_depth<2>(F, lane, tab);
+        FP::template coeff_depth<3>(F, lane, tab);
+        FP::template coeff_depth<4>(F, lane, tab);
+        FP::template coeff_depth<5>(F, lane, tab);
+        FP::template coeff_depth<6>(F, lane, tab);
+        if constexpr (LOG == 8) FP::template coeff_depth<7>(F, lane, tab);
         // leaves per ciphertext: leaf[x][p]
         uint16_t *leaf = reinterpret_cast<uint16_t *>(lds);
 #pragma unroll
@@ -693,199 +977,12 @@ struct FusedTileKernel {
         wave_sync();
     }
 
-    // ---- phase B data layout --------------------------------------------------------------
-    // Word w (32 evaluation points) of a ciphertext lives in lane l, register r.  Its index bits
-    // start as w = l + 64 r.  Butterfly level J pairs words that differ in index bit J; when that
-    // bit is a lane bit, it is first exchanged with a register bit (a 2 x 2 transpose between
-    // lane pairs), so every butterfly runs inside one lane on half the words: no product is
-    // computed twice.  The exchange schedule is fixed at compile time: when bit J is in a lane,
-    // it replaces the register bit whose index is needed furthest in the future (or never).
-    static constexpr int RBITS = WPL == 4 ? 2 : 1;      // register bits of the word index
-    static constexpr int NLEV = LOG;                    // butterfly levels J = JMIN .. JMIN+LOG-1
-    static constexpr int JMIN = M - 5 - LOG;
-    struct Layout {
-        int posr[2];   // word-index bit held by register bit q
-        int posl[6];   // word-index bit held by lane bit i
-    };
-    struct Schedule {
-        int swap_lane[8];   // lane bit exchanged with register bit swap_q[i] before level i (-1: none)
-        int swap_q[8];
-        int bq[8];          // register bit holding bit J at level i
-        Layout at[8];       // layout during level i
-        Layout fin;         // layout after the last level
-    };
-    static constexpr Schedule make_schedule()
-    {
-        Schedule sc{};
-        Layout cur{};
-        for (int q = 0; q < 2; q++) cur.posr[q] = q < RBITS ? 6 + q : -1;
-        for (int i = 0; i < 6; i++) cur.posl[i] = i;
-        for (int i = 0; i < NLEV; i++) {
-            const int J = JMIN + i;
-            int q_hold = -1;
-            for (int q = 0; q < RBITS; q++)
-                if (cur.posr[q] == J) q_hold = q;
-            sc.swap_lane[i] = -1;
-            sc.swap_q[i] = -1;
-            if (q_hold < 0) {
-                int lb = 0;
-                for (int b = 0; b < 6; b++)
-                    if (cur.posl[b] == J) lb = b;
-                int best = 0, key = -1;
-                for (int q = 0; q < RBITS; q++) {
-                    const int k = cur.posr[q] < J ? 1000 : cur.posr[q];
-                    if (k > key) { key = k; best = q; }
-                }
-                sc.swap_lane[i] = lb;
-                sc.swap_q[i] = best;
-                cur.posl[lb] = cur.posr[best];
-                cur.posr[best] = J;
-                q_hold = best;
-            }
-            sc.bq[i] = q_hold;
-            sc.at[i] = cur;
-        }
-        sc.fin = cur;
-        return sc;
-    }
-    static constexpr Schedule SCHED = make_schedule();
-
-    // Index bits of lane l's words that come from the lane: bits of l placed where the layout
-    // says, restricted to word-index bits below LIMIT.
-    static __device__ __forceinline__ uint32_t lane_part(const Layout &lay, int lane, int limit)
-    {
-        uint32_t w = 0;
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-            if (lay.posl[i] < limit) w |= (uint32_t)((lane >> i) & 1) << lay.posl[i];
-        return w;
-    }
-    static constexpr uint32_t reg_part(const Layout &lay, int r, int limit)
-    {
-        uint32_t w = 0;
-        for (int q = 0; q < RBITS; q++)
-            if (lay.posr[q] < limit) w |= (uint32_t)((r >> q) & 1) << lay.posr[q];
-        return w;
-    }
-
-    // Exchanges lane bit LB with register bit Q: lane l keeps the words whose register bit Q
-    // equals its lane bit LB and trades the others with lane l ^ 2^LB.
-    template <int LB, int Q>
-    static __device__ __forceinline__ void swap_bits(uint32_t (&W)[WPL][M], int lane)
-    {
-        const bool upper = (opaque(lane) >> LB) & 1;
-#pragma unroll
-        for (int r0 = 0; r0 < WPL; r0++) {
-            if ((r0 >> Q) & 1) continue;
-            const int r1 = r0 | (1 << Q);
-#pragma unroll
-            for (int k = 0; k < M; k++) {
-                const uint32_t send = upper ? W[r0][k] : W[r1][k];
-                const uint32_t recv = xor_lanes<(1 << LB)>(send);
-                W[r0][k] = upper ? recv : W[r0][k];
-                W[r1][k] = upper ? W[r1][k] : recv;
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-
-    // Butterfly level Dl (word-index bit J = M - 6 - Dl): w0 = u + G v, w1 = w0 + v, where
-    // G = lowpat (by bit position) + hi (by the low J bits of the lower word's index).
-    template <int Dl>
-    static __device__ __forceinline__ void butterfly(uint32_t (&W)[WPL][M], int lane, const FftTab *tab)
-    {
-        constexpr int J = M - 6 - Dl;
-        constexpr int LI = J - JMIN;
-        constexpr int SL = SCHED.swap_lane[LI], SQ = SCHED.swap_q[LI];
-        if constexpr (SL >= 0) swap_bits<(SL >= 0 ? SL : 0), (SQ >= 0 ? SQ : 0)>(W, lane);
-        constexpr int RB = 1 << SCHED.bq[LI];
-        constexpr Layout LAY = SCHED.at[LI];
-        // Reload this level's constants here: hoisted out of the per-ciphertext loop, the
-        // multipliers of all levels would not fit in registers and would go to scratch.
-        asm volatile("" ::: "memory");
-        uint32_t lowp[M];
-#pragma unroll
-        for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
-        const uint32_t wl = lane_part(LAY, opaque(lane), J);
-#pragma unroll
-        for (int r = 0; r < WPL; r++) {
-            if (r & RB) continue;
-            const uint32_t hv = tab->hi[Dl][wl | reg_part(LAY, r, J)];
-            uint32_t G[M], t[M];
-#pragma unroll
-            for (int k = 0; k < M; k++) G[k] = lowp[k] ^ bitmask(hv, k);
-            bs_mul<M>(G, W[r | RB], t);
-#pragma unroll
-            for (int k = 0; k < M; k++) {
-                W[r][k] ^= t[k];
-                W[r | RB][k] ^= W[r][k];
-            }
-            // one word pair at a time: interleaving the products would multiply the live
-            // registers
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-
-    // Full word index of lane l's register r in the final layout.
-    static __device__ __forceinline__ uint32_t final_word(uint32_t wl_fin, int r)
-    {
-        return wl_fin | reg_part(SCHED.fin, r, 64);
-    }
-
-    template <int K>
-    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], gu32 *masks, int lane,
-                                                       uint32_t wl_fin)
-    {
-        constexpr int S = K < M ? K : 2 * M - 2 - K;
-        gu32 *mrow = masks + K * NW + opaque((int)wl_fin);
-        uint32_t mw[WPL];
-#pragma unroll
-        for (int r = 0; r < WPL; r++) mw[r] = mrow[reg_part(SCHED.fin, r, 64)];
-        if constexpr (S < 5) {
-            constexpr int SH = 1 << S;
-#pragma unroll
-            for (int r = 0; r < WPL; r++) {
-                const uint32_t t = ((z[r] >> SH) ^ z[r]) & mw[r];
-                z[r] ^= t ^ (t << SH);
-            }
-        } else {
-            constexpr int JB = S - 5;  // word-index bit
-            constexpr int QB = SCHED.fin.posr[0] == JB ? 0 : (RBITS > 1 && SCHED.fin.posr[1] == JB ? 1 : -1);
-            if constexpr (QB >= 0) {
-                constexpr int RB = 1 << QB;
-#pragma unroll
-                for (int r = 0; r < WPL; r++) {
-                    if (r & RB) continue;
-                    const uint32_t d = (z[r] ^ z[r | RB]) & mw[r];
-                    z[r] ^= d;
-                    z[r | RB] ^= d;
-                }
-            } else {
-                constexpr int LBB = SCHED.fin.posl[0] == JB ? 0 : SCHED.fin.posl[1] == JB ? 1
-                                  : SCHED.fin.posl[2] == JB ? 2 : SCHED.fin.posl[3] == JB ? 3
-                                  : SCHED.fin.posl[4] == JB ? 4 : 5;
-#pragma unroll
-                for (int r = 0; r < WPL; r++) {
-                    const uint32_t other = xor_lanes<(1 << LBB)>(z[r]);
-                    z[r] ^= (z[r] ^ other) & mw[r];
-                }
-            }
-        }
-    }
-
-    template <int... Ks>
-    static __device__ __forceinline__ void benes_all(uint32_t (&z)[WPL], gu32 *masks, int lane,
-                                                     uint32_t wl_fin, std::integer_sequence<int, Ks...>)
-    {
-        (benes_layer<Ks>(z, masks, lane, wl_fin), ...);
-    }
-
     static __device__ __forceinline__ void roots_fft(const FusedArgs &a, const FusedTile &tl, const uint32_t *lds,
                                                      int lane, bool e_aligned, const FftTab *tab)
     {
         const uint16_t *leaf = reinterpret_cast<const uint16_t *>(lds);
         for (int x = 0; x < tl.count; x++) {
-            const uint32_t wl_fin = lane_part(SCHED.fin, opaque(lane), 64);
+            const uint32_t wl_fin = FP::lane_part(FP::SCHED.fin, opaque(lane), 64);
             const int item = a.items[tl.first + x];
             const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
             uint8_t *eo = a.e + (size_t)item * a.e_stride;
@@ -897,14 +994,14 @@ struct FusedTileKernel {
 #pragma unroll
                 for (int k = 0; k < M; k++) W[r][k] = bitmask(v, k);
             }
-            if constexpr (LOG == 8) butterfly<7>(W, lane, tab);
-            butterfly<6>(W, lane, tab);
-            butterfly<5>(W, lane, tab);
-            butterfly<4>(W, lane, tab);
-            butterfly<3>(W, lane, tab);
-            butterfly<2>(W, lane, tab);
-            butterfly<1>(W, lane, tab);
-            butterfly<0>(W, lane, tab);
+            if constexpr (LOG == 8) FP::template butterfly<7>(W, lane, tab);
+            FP::template butterfly<6>(W, lane, tab);
+            FP::template butterfly<5>(W, lane, tab);
+            FP::template butterfly<4>(W, lane, tab);
+            FP::template butterfly<3>(W, lane, tab);
+            FP::template butterfly<2>(W, lane, tab);
+            FP::template butterfly<1>(W, lane, tab);
+            FP::template butterfly<0>(W, lane, tab);
             uint32_t z[WPL];
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
@@ -913,10 +1010,10 @@ struct FusedTileKernel {
                 for (int k = 0; k < M; k++) nz |= W[r][k];
                 z[r] = ~nz;
             }
-            benes_all(z, gptr(kd->benes), lane, wl_fin, std::make_integer_sequence<int, 2 * M - 1>{});
+            FP::benes_all(z, gptr(kd->benes), lane, wl_fin, std::make_integer_sequence<int, 2 * M - 1>{});
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
-                const int g = (int)final_word(wl_fin, r);
+                const int g = (int)FP::final_word(wl_fin, r);
                 if (g >= G) continue;
                 const int nbytes = (N / 8 - 4 * g) < 4 ? (N / 8 - 4 * g) : 4;
                 uint8_t *dst = eo + 4 * g;

@@ -13,6 +13,8 @@ struct KeyDev {
     const uint16_t *L;      // n support elements
     const uint32_t *lbits;  // m x lbits_stride bit planes of L (32 positions per word)
     const uint32_t *benes;  // (2m-1) x 2^(m-5) swap masks of the key's Benes network, per 32-bit word
+    const uint32_t *ypl;    // m x 2^(m-5) bit planes of 1/g(a)^2 in FFT order (word w, bit b: a = bitrev_m(32w+b))
+    const uint8_t *s;       // n/8 bytes of the secret key's rejection value s
     int32_t pidx;
     int32_t lstride;        // words per bit-plane row
 };
@@ -30,7 +32,7 @@ constexpr int fft_variant(int m, int l) { return m == 12 ? 0 : (l == 7 ? 1 : 2);
 
 // ---- on-device key expansion ---------------------------------------------------------
 // d_key holds the secret key from byte 40 on: t little-endian Goppa coefficients, then the
-// Benes control bits.  Fills k.L, k.lbits, k.table and k.benes (all of set pidx).
+// Benes control bits.  Fills k.L, k.lbits, k.table, k.benes and k.ypl (all of set pidx).
 hipError_t launch_key_expand(int pidx, const uint8_t *d_key, const KeyDev &k, hipStream_t st);
 
 // ---- plain per-stage pipeline (one key per launch group) ----------------------------
@@ -72,6 +74,48 @@ struct FusedSchedule {
     int groups_per_cu;    // persistent workgroups per CU
     int root_alg;         // 0: Horner at each support point, 1: additive FFT + Benes network
 };
+
+// ---- KEM decapsulation -----------------------------------------------------------------
+// Re-encryption check of decoded error vectors, over the tiles of a fused plan.  For item x,
+// v = (c || 0) ^ e (or v = e alone when ct is null) and the check computes the power sums
+// s_j(v) = sum_{i < n} v_i L_i^j / g(L_i)^2, j < 2t, by the transposed additive FFT.
+//   ok != null:    ok[item] = 1 if weight(e) == t and s(v) == 0, else 0
+//   s_out != null: s_out[item * 2t + j] = s_j(v)
+struct DecapArgs {
+    const FusedTile *tiles;
+    int32_t n_tiles;
+    int32_t pad0;
+    const int32_t *items;
+    const int32_t *item_key;
+    const KeyDev *keys;
+    const uint8_t *ct;         // ciphertexts, or null
+    const uint8_t *e;          // error vectors (n/8 bytes each)
+    uint64_t ct_stride, e_stride;
+    uint32_t *ok;
+    uint16_t *s_out;
+    const FftTab *fft;
+};
+hipError_t launch_decap_check(const DecapArgs &a, hipStream_t st);
+
+// Session keys: per item, SHAKE256(b || (b ? e : s) || c) truncated to 32 bytes, b = ok[item]
+// (crypto_kem_dec).  Slot q < count is item items[q] of key slot item_key[q] (the arrays of a
+// fused plan); msg is scratch of count * kKemMsgStride bytes.
+constexpr int kKemMsgStride = 1360;  // 10 SHAKE256 blocks: the largest padded preimage (8192128)
+struct KemHashArgs {
+    int32_t count;
+    int32_t pad0;
+    const int32_t *items;
+    const int32_t *item_key;
+    const KeyDev *keys;
+    const uint8_t *ct;
+    const uint8_t *e;
+    uint64_t ct_stride, e_stride;
+    const uint32_t *ok;
+    uint8_t *msg;
+    uint8_t *key_out;
+    uint64_t key_stride;
+};
+hipError_t launch_kem_hash(const KemHashArgs &a, hipStream_t st);
 
 // Host: fills the FFT constants for (m, l).
 void build_fft_tab(int m, int l, FftTab *out);

@@ -7,7 +7,9 @@
 //            table[i][j] = L_i^j / g(L_i)^2 for i < sb, j < 2t;
 //   lbits    bit planes of L, 32 support positions per word;
 //   benes    per layer and 32-bit word, the positions whose switch is set (for strides of 32
-//            and more both words of a pair carry the pair's bit).
+//            and more both words of a pair carry the pair's bit);
+//   ypl      bit planes of 1/g(a)^2 at every field element a in FFT order (word w, bit b holds
+//            a = bitrev_m(32w + b)), the weights of the decapsulation check's syndrome.
 #include <hip/hip_runtime.h>
 
 #include "dev_gf.h"
@@ -102,6 +104,31 @@ __global__ __launch_bounds__(256) void k_key_table(const uint8_t *__restrict__ g
     }
 }
 
+// One thread per field element c (FFT order): y = 1/g(bitrev_m(c))^2, stored as m bit planes.
+template <int IDX>
+__global__ __launch_bounds__(256) void k_key_ypl(const uint8_t *__restrict__ gbytes, uint32_t *ypl)
+{
+    using Pm = P<IDX>;
+    constexpr int M = Pm::m, T = Pm::t, WORDS = 1 << (M - 5);
+    constexpr uint32_t MASK = (1u << M) - 1u;
+    const int c = blockIdx.x * 256 + threadIdx.x;  // the grid covers 2^m exactly
+    const int lane = threadIdx.x & 63;
+    const uint32_t a = __builtin_bitreverse32((uint32_t)c) >> (32 - M);
+    uint32_t ev = 1;
+    for (int i = T - 1; i >= 0; i--) {
+        const uint32_t gi = ((uint32_t)gbytes[2 * i] | ((uint32_t)gbytes[2 * i + 1] << 8)) & MASK;
+        ev = gf_mul<M>(ev, a) ^ gi;
+    }
+    const uint32_t y = gf_inv<M>(gf_mul<M>(ev, ev));
+    const int w = 2 * (c >> 6);
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const uint64_t ball = __ballot((y >> k) & 1u);
+        if (lane == 0) ypl[k * WORDS + w] = (uint32_t)ball;
+        if (lane == 1) ypl[k * WORDS + w + 1] = (uint32_t)(ball >> 32);
+    }
+}
+
 // One thread per (layer, 32-bit word) of the swap masks.
 template <int IDX>
 __global__ __launch_bounds__(256) void k_key_benes(const uint8_t *__restrict__ cond, uint32_t *benes)
@@ -139,6 +166,9 @@ hipError_t launch_expand(const uint8_t *d_key, const KeyDev &k, hipStream_t st)
     const int mask_words = (2 * Pm::m - 1) << (Pm::m - 5);
     hipLaunchKernelGGL(k_key_benes<IDX>, dim3((mask_words + 255) / 256), dim3(256), 0, st, cond,
                        const_cast<uint32_t *>(k.benes));
+    static_assert(((1 << Pm::m) % 256) == 0, "whole blocks of field elements");
+    hipLaunchKernelGGL(k_key_ypl<IDX>, dim3((1 << Pm::m) / 256), dim3(256), 0, st, gbytes,
+                       const_cast<uint32_t *>(k.ypl));
     return hipGetLastError();
 }
 

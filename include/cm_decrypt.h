@@ -112,6 +112,9 @@ typedef struct cmd_ctx cmd_ctx;
 
 #define CMD_MODE_PLAIN 0   /* three launches per key group: synd, bm, chien (uint16 GF) */
 #define CMD_MODE_FUSED 1   /* one persistent synd->bm->chien launch for any mix of sets/keys */
+#define CMD_MODE_KEM   2   /* fused decryption + decapsulation check + SHAKE256: the output row
+                              of an item is its 32-byte session key (crypto_kem_dec) */
+#define CMD_KEM_KEY_BYTES 32
 
 /* device: HIP ordinal; hbm_budget_bytes: 0 = 90% of the free device memory. */
 int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx **out);
@@ -148,6 +151,17 @@ int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids
 int cmd_decrypt_device(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
                        const void *d_ct, size_t ct_stride, void *d_e, size_t e_stride, void *stream);
 
+/* KEM decapsulation, host buffers (crypto_kem_dec of Classic McEliece, batched).  Item i:
+ * ciphertext ct + i*ct_stride of key key_ids[i]; its 32-byte session key goes to
+ * key_out + i*key_stride.  The key is SHAKE256(1 || e || c) when the decoded e has weight t
+ * and the same syndrome over the whole support as c, else SHAKE256(0 || s || c) with the
+ * secret key's s (implicit rejection; both cases run the same instructions).  The success key
+ * is the one the reference's encapsulation derives (kem/common/core/operations.c:15-34);
+ * decapsulation itself has no counterpart in the reference, which stops at e.  Same as
+ * cmd_decrypt with CMD_MODE_KEM; cmd_decrypt_device and batches take that mode too. */
+int cmd_kem_dec(cmd_ctx *ctx, size_t count, const int32_t *key_ids, const uint8_t *ct, size_t ct_stride,
+                uint8_t *key_out, size_t key_stride);
+
 /* ---- prepared batches (fixed key assignment, replayed as one hipGraph) --------------- */
 typedef struct cmd_batch cmd_batch;
 
@@ -168,6 +182,12 @@ int cmd_stage_bm(cmd_ctx *ctx, int32_t param_id, size_t count, const uint16_t *s
 /* root + build_error_vector (root_13.c:233-240, decrypt.c:43-53): loc: count x (t+1). */
 int cmd_stage_root(cmd_ctx *ctx, int32_t key_id, size_t count, const uint16_t *loc, uint8_t *e_out,
                    size_t e_stride);
+/* Syndrome of n-bit words over the whole support, s_j(r) = sum_{i<n} r_i L_i^j / g(L_i)^2 for
+ * j < 2t (the decapsulation check's s_cmp; the reference's synd, util.c:102-114, covers the
+ * first sb positions, where the two agree).  r: count rows of n/8 bytes at r_stride; s_out:
+ * count x 2t uint16.  Computed by the transposed additive FFT of CMD_MODE_KEM. */
+int cmd_stage_synd_full(cmd_ctx *ctx, int32_t key_id, size_t count, const uint8_t *r, size_t r_stride,
+                        uint16_t *s_out);
 
 /* ---- schedules (autotuner) ----------------------------------------------------------- */
 /* Fused-kernel schedule per set: ciphertexts per wavefront tile (16/32), waves per workgroup,

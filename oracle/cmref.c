@@ -356,6 +356,24 @@ int orc_synd(int id, const uint16_t *table, const uint8_t *ct, uint16_t *s)
     return 0;
 }
 
+int orc_synd_full(int id, const uint16_t *g, const uint16_t *L, const uint8_t *r, uint16_t *s)
+{
+    orc_params p;
+    if (orc_param(id, &p)) return -1;
+    const int w = 2 * p.t;
+    memset(s, 0, sizeof(uint16_t) * w);
+    for (int i = 0; i < p.n; i++) {
+        if (!((r[i >> 3] >> (i & 7)) & 1u)) continue;
+        uint16_t v = eval_poly(p.m, p.t, g, L[i]);
+        uint16_t c = orc_gf_inv(p.m, orc_gf_mul(p.m, v, v));
+        for (int j = 0; j < w; j++) {
+            s[j] ^= c;
+            c = orc_gf_mul(p.m, c, L[i]);
+        }
+    }
+    return 0;
+}
+
 /*
  * Constant-time Berlekamp-Massey, CPU/src/common/bm.c:12-68, step for step:
  * C = 1, B = x, b = 1, L = 0; for N < 2t: d = sum_{i<=min(N,t)} C_i s_{N-i};

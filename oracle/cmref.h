@@ -24,6 +24,16 @@
  *                   on random (non-codeword) syndromes.
  *   orc_root        Horner closed form; end to end recovered e == generated e.
  *   orc_errorstream byte format vs. the reference executable's output file.
+ *   orc_synd_full   equals orc_synd (the reference's table) on words inside the first
+ *                   sb positions; s(e) == s(c) for every generated ciphertext c of
+ *                   error e (the code's defining relation, through the public key).
+ *
+ * Decapsulation (tests/_oracle.py kem_dec) composes these with SHAKE256 from Python's
+ * hashlib (FIPS 202).  Its success key is the reference encapsulation's formula
+ * SHAKE256(1 || e || c) (kem/common/core/operations.c:15-34); the rejection key
+ * SHAKE256(0 || s || c) and the acceptance test (weight t, equal syndromes) follow the
+ * published Classic McEliece specification's crypto_kem_dec, which the reference does not
+ * implement: that branch has no reference counterpart to pin it against.
  */
 #ifndef CMREF_H
 #define CMREF_H
@@ -73,6 +83,13 @@ int orc_synd(int id, const uint16_t *table, const uint8_t *ct, uint16_t *s);    
 int orc_bm(int id, const uint16_t *s, uint16_t *locator);                        /* locator[t+1], reversed */
 int orc_root(int id, const uint16_t *locator, const uint16_t *L, uint16_t *images); /* images[n] */
 int orc_error_vector(int id, const uint16_t *images, uint8_t *e);                /* e[n/8] */
+
+/*
+ * Syndrome of an n-bit word over the whole support (the specification's synd over all n
+ * positions): s[j] = XOR over i < n with bit i of r set of L_i^j / g(L_i)^2, j < 2t.
+ * g: t+1 coefficients (g[t] = 1), L: n support elements (orc_key_expand).
+ */
+int orc_synd_full(int id, const uint16_t *g, const uint16_t *L, const uint8_t *r, uint16_t *s);
 
 /* Whole decryption of `count` ciphertexts (ct_stride bytes apart) under one key. */
 int orc_decrypt(int id, const uint8_t *sk, const uint8_t *ct, size_t ct_stride,

@@ -91,6 +91,47 @@ def synd(pid, table, ct):
     return s
 
 
+def synd_full(pid, g, L, r):
+    """s_j(r) = sum_{i<n} r_i L_i^j / g(L_i)^2, j < 2t (all n support positions)."""
+    t = PARAMS[pid][2]
+    s = np.zeros(2 * t, np.uint16)
+    assert lib().orc_synd_full(pid, _p(np.ascontiguousarray(g, np.uint16)), _p(np.ascontiguousarray(L, np.uint16)),
+                               _p(np.ascontiguousarray(r, np.uint8)), _p(s)) == 0
+    return s
+
+
+COND_BYTES = {348864: 5888, 460896: 12800, 6688128: 12800, 6960119: 12800, 8192128: 12800}
+
+
+def shake256_32(data):
+    import hashlib
+    return hashlib.shake_256(bytes(data)).digest(32)
+
+
+def kem_dec(pid, sk, ct):
+    """crypto_kem_dec of the Classic McEliece specification for ciphertext rows under one key:
+    (session keys count x 32, accepted flags).  e from the oracle decoder is accepted when its
+    weight is t and s(e) over all n positions equals the ciphertext's syndrome; the key is
+    SHAKE256(1 || e || c) then (the reference encapsulation's formula,
+    kem/common/core/operations.c:15-34), else SHAKE256(0 || s || c) with the secret key's s."""
+    m, n, t, ctb, sb, skb = PARAMS[pid]
+    sk = np.ascontiguousarray(sk, np.uint8)
+    ct = np.ascontiguousarray(ct, np.uint8).reshape(-1, ctb)
+    g, L, T = key_expand(pid, sk)
+    s_rej = sk[40 + 2 * t + COND_BYTES[pid]:][:n // 8]
+    E = decrypt(pid, sk, ct)
+    keys = np.zeros((ct.shape[0], 32), np.uint8)
+    ok = np.zeros(ct.shape[0], bool)
+    for i in range(ct.shape[0]):
+        e = E[i]
+        w = int(np.unpackbits(e).sum())
+        good = w == t and np.array_equal(synd(pid, T, ct[i]), synd_full(pid, g, L, e))
+        pre = (bytes([1]) + e.tobytes() if good else bytes([0]) + s_rej.tobytes()) + ct[i].tobytes()
+        keys[i] = np.frombuffer(shake256_32(pre), np.uint8)
+        ok[i] = good
+    return keys, ok
+
+
 def bm(pid, s):
     t = PARAMS[pid][2]
     out = np.zeros(t + 1, np.uint16)

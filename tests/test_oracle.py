@@ -323,3 +323,66 @@ def test_late_first_syndrome_vs_reference_build(pid, K):
         assert np.array_equal(s, ref.synd(c))
         assert np.array_equal(orc.bm(pid, s), ref.bm(s))
     assert np.array_equal(orc.decrypt(pid, sk, ct), ref.decrypt(ct))
+
+
+# ---- decapsulation pieces -------------------------------------------------------------------
+
+def test_shake256_known_answer():
+    """SHAKE256 of the empty message, 256 output bits (FIPS 202 example value)."""
+    assert orc.shake256_32(b"").hex() == "46b9dd2b0ba88d13233b3feb743eeb243fcd52ea62b81b82b50c27646ed5762f"
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_synd_full_is_reference_synd_on_ciphertext_bits(pid):
+    """On words inside the first sb positions the full-support syndrome equals the reference's
+    table-driven synd (util.c:102-114)."""
+    m, n, t, ctb, sb, _ = orc.PARAMS[pid]
+    g, L, T = orc.key_expand(pid, orc.shipped_sk(pid))
+    rng = np.random.default_rng(pid)
+    for _ in range(3):
+        ct = rng.integers(0, 256, ctb, dtype=np.uint8)
+        r = np.zeros(n // 8, np.uint8)
+        r[:ctb] = ct
+        assert np.array_equal(orc.synd_full(pid, g, L, r), orc.synd(pid, T, ct))
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_synd_full_codeword_relation(pid):
+    """c = H e for the golden ciphertexts: s(e) over all n positions equals s(c); a flipped bit
+    of e outside the first sb positions breaks the equality."""
+    m, n, t, ctb, sb, _ = orc.PARAMS[pid]
+    gold = np.load(os.path.join(orc.GOLDEN, f"golden_{pid}.npz"))
+    g, L, T = orc.key_expand(pid, orc.shipped_sk(pid))
+    for c, e in zip(gold["ct"], gold["e"]):
+        assert np.array_equal(orc.synd_full(pid, g, L, e), orc.synd(pid, T, c))
+    e2 = gold["e"][0].copy()
+    e2[-1] ^= 0x80
+    assert not np.array_equal(orc.synd_full(pid, g, L, e2), orc.synd(pid, T, gold["ct"][0]))
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_kem_dec_oracle(pid):
+    """Valid ciphertexts give the reference encapsulation's key SHAKE256(1 || e || c)
+    (kem/common/core/operations.c:15-34); random ciphertexts, a flipped ciphertext bit and
+    errors of weight t-1 are rejected with SHAKE256(0 || s || c)."""
+    m, n, t, ctb, sb, _ = orc.PARAMS[pid]
+    sk = orc.shipped_sk(pid)
+    gold = np.load(os.path.join(orc.GOLDEN, f"golden_{pid}.npz"))
+    keys, ok = orc.kem_dec(pid, sk, gold["ct"][:3])
+    assert ok.all()
+    for c, e, k in zip(gold["ct"], gold["e"], keys):
+        assert k.tobytes() == orc.shake256_32(b"\x01" + e.tobytes() + c.tobytes())
+    # a ciphertext bit that is also an error position: decodes to weight t-1
+    c0, e0 = gold["ct"][0].copy(), gold["e"][0]
+    i = int(np.flatnonzero(np.unpackbits(e0[:ctb], bitorder="little"))[0])
+    low = c0.copy()
+    low[i >> 3] ^= 1 << (i & 7)
+    flip = c0.copy()
+    flip[0] ^= 0x01 if not (e0[0] & 1) else 0x02
+    rnd = np.random.default_rng(pid).integers(0, 256, (2, ctb), dtype=np.uint8)
+    bad = np.vstack([low, flip, rnd])
+    keys, ok = orc.kem_dec(pid, sk, bad)
+    assert not ok.any()
+    s_rej = sk[40 + 2 * t + orc.COND_BYTES[pid]:][:n // 8].tobytes()
+    for c, k in zip(bad, keys):
+        assert k.tobytes() == orc.shake256_32(b"\x00" + s_rej + c.tobytes())
