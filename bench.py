@@ -5,7 +5,8 @@ Workload (the headline metric): mceliece8192128, 16 seeded keys, 65536 ciphertex
 assigned round-robin to the keys, each with an exactly-weight-128 random error.  A step
 is one decryption of the whole batch by the fused persistent kernel (one hipGraph launch),
 with ciphertexts already resident in HBM.  The plain per-stage pipeline (three launches per
-key group) runs the same batch for the fused/plain ratio; a mixed-parameter Poisson
+key group) runs the same batch for the fused/plain ratio; KEM decapsulation (decryption,
+re-encryption check and SHAKE256 session key) runs it as crypto_kem_dec; a mixed-parameter Poisson
 request stream through the serving loop gives p50/p99 latency; the reference CPU decoder
 (built from its own C sources) is timed on a bounded sample on this host.
 
@@ -13,6 +14,7 @@ Prints one JSON line.  Only one GPU is used; --gpus above 1 is refused.
 """
 import argparse
 import concurrent.futures as cf
+import hashlib
 import json
 import re
 import os
@@ -100,6 +102,23 @@ def c2_leg(cm, torch, ctx, steps, threads):
                      "bit_exact": bool(np.array_equal(d_e.cpu().numpy(), e_true))}
     ctx.unload_key(kid)
     return out
+
+
+def kem_leg(cm, torch, ctx, key_ids, d_ct, ct, e_true, steps, warmup, stream):
+    """crypto_kem_dec on the headline workload: the fused decryption, the transposed-FFT
+    re-encryption check and SHAKE256 as one hipGraph.  Every ciphertext is valid, so each key
+    must be SHAKE256(1 || e || c), the reference encapsulation's key (checked with hashlib)."""
+    n = len(key_ids)
+    d_k = torch.zeros((n, 32), dtype=torch.uint8, device="cuda")
+    b = cm.Batch(ctx, key_ids, ct.shape[1], 32, mode=cm.MODE_KEM, use_graph=True)
+    with torch.cuda.stream(stream):
+        wall, kern_ms = time_batch(torch, b, d_ct, d_k, steps, warmup)
+    b.close()
+    keys = d_k.cpu().numpy()
+    exact = all(keys[i].tobytes() == hashlib.shake_256(b"\x01" + e_true[i].tobytes() + ct[i].tobytes()).digest(32)
+                for i in range(n))
+    return {"value": round(n * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+            "kernel_ms": round(kern_ms, 4), "bit_exact": bool(exact)}
 
 
 def key_pool(cm, ctx, n_keys, seed, threads):
@@ -340,6 +359,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-kem", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serve-keys", type=int, default=1024)
     ap.add_argument("--serve-rates", default="1e4,1e5,3e5,1e6")
@@ -404,6 +424,13 @@ def main():
     log(f"fused: {value:,.0f} dec/s, {wall / args.steps * 1e3:.3f} ms/step, kernel {kern_ms:.3f} ms")
     e_fused = d_e.cpu().numpy()
     exact_fused = bool(np.array_equal(e_fused, e_true))
+
+    kem = None
+    if not args.no_kem:
+        kem = kem_leg(cm, torch, ctx, key_ids, d_ct, ct, e_true, args.steps, args.warmup, stream)
+        kem["decap_overhead_ms"] = round(kem["kernel_ms"] - kern_ms, 4)
+        log(f"kem_dec: {kem['value']:,.0f} decaps/s, graph {kem['kernel_ms']:.3f} ms "
+            f"(+{kem['decap_overhead_ms']:.3f} ms over decryption) exact={kem['bit_exact']}")
 
     plain = None
     if not args.no_plain:
@@ -481,6 +508,7 @@ def main():
                              "measured v_xor_b32 issue rate of this GPU (nominal 256 CU x 64 lanes x 2.4 GHz)"},
         "roofline_hbm": {"bound": "hbm", "achieved": round(hbm_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
+        "kem_dec": kem,
         "plain_6960119_4096": c2,
         "mixed_batch": mixed,
         "serve": serve,
