@@ -6,7 +6,7 @@
 // SHAKE256(0 || s || c) with the secret key's s (implicit rejection), selecting by mask so that
 // the two cases run the same instructions.
 //
-// k_decap_check: one wavefront per tile of a fused plan.  By linearity the check is s(v) == 0
+// k_decap_check: one workgroup per tile of a fused plan.  By linearity the check is s(v) == 0
 // for v = (c || 0) ^ e, where s_j(v) = sum_{i < n} v_i L_i^j / g(L_i)^2 (j < 2t).  Per
 // ciphertext, v is taken from support order to FFT order by the key's Benes network in reverse,
 // multiplied by the weights 1/g(a)^2 (key planes in FFT order), and reduced to the 2t power
@@ -14,7 +14,7 @@
 // coefficient phase in reverse, bit-sliced across the tile.  Cost is that of the root search's
 // FFT, against n * 2t multiply-adds for the direct sum.
 //
-// k_kem_pack / k_keccak: the padded SHAKE256 input per item (one wavefront per item, byte
+// k_kem_pack / k_keccak: the padded SHAKE256 input per item (one wavefront per item, 64-bit
 // stores), then Keccak-f[1600] per item (one lane per item).
 #include "fused_impl.h"
 
@@ -58,15 +58,18 @@ struct DecapTile {
         return v;
     }
 
-    static __device__ __forceinline__ void run(const DecapArgs &a, const FusedTile &tl, uint32_t *lds, int lane)
+    // The tile's wavefronts share the per-ciphertext part (slot x goes to wavefront x mod
+    // nwaves); wavefront 0 then runs the coefficient phase over the leaves in LDS.  *wbits
+    // (LDS, zero on entry) collects the weight bits.
+    static __device__ __forceinline__ void run(const DecapArgs &a, const FusedTile &tl, uint32_t *lds,
+                                               uint32_t *wbits, int lane, int wave, int nwaves)
     {
         const FftTab *tab = a.fft + fft_variant(M, LOG);
         uint16_t *leaf = reinterpret_cast<uint16_t *>(lds);
         const bool e_al = ((a.e_stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(a.e) & 3u) == 0);
         const bool c_al = ((a.ct_stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(a.ct) & 3u) == 0);
         uint32_t weight_ok = 0;  // bit x: e of slot x has weight t (wave-uniform)
-        wave_sync();
-        for (int x = 0; x < tl.count; x++) {
+        for (int x = wave; x < tl.count; x += nwaves) {
             const uint32_t wl_fin = FP::lane_part(FP::SCHED.fin, opaque(lane), 64);
             const int item = a.items[tl.first + x];
             const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
@@ -109,7 +112,10 @@ struct DecapTile {
                 leaf[x * (1 << LOG) + bitrev_bits((uint32_t)w, LOG)] = (uint16_t)v;
             }
         }
-        wave_sync();
+        if (lane == 0 && weight_ok) atomicOr(wbits, weight_ok);
+        __syncthreads();
+        if (wave != 0) return;
+        weight_ok = uni(*wbits);
         // coefficient phase in reverse, bit-sliced across the tile (bit x = slot x)
         uint32_t F[NP][M];
 #pragma unroll
@@ -161,17 +167,22 @@ struct DecapTile {
     }
 };
 
-__global__ __launch_bounds__(64) void k_decap_check(DecapArgs a)
+constexpr int kDecapWaves = 4;  // wavefronts per tile (workgroup)
+
+__global__ __launch_bounds__(64 * kDecapWaves) void k_decap_check(DecapArgs a)
 {
     __shared__ uint32_t lds[32 * 256 / 2];
+    __shared__ uint32_t wbits;
     const FusedTile tl = a.tiles[blockIdx.x];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) wbits = 0;
+    __syncthreads();
     switch (tl.pidx) {
-    case 0: DecapTile<0>::run(a, tl, lds, lane); break;
-    case 1: DecapTile<1>::run(a, tl, lds, lane); break;
-    case 2: DecapTile<2>::run(a, tl, lds, lane); break;
-    case 3: DecapTile<3>::run(a, tl, lds, lane); break;
-    case 4: DecapTile<4>::run(a, tl, lds, lane); break;
+    case 0: DecapTile<0>::run(a, tl, lds, &wbits, lane, wave, kDecapWaves); break;
+    case 1: DecapTile<1>::run(a, tl, lds, &wbits, lane, wave, kDecapWaves); break;
+    case 2: DecapTile<2>::run(a, tl, lds, &wbits, lane, wave, kDecapWaves); break;
+    case 3: DecapTile<3>::run(a, tl, lds, &wbits, lane, wave, kDecapWaves); break;
+    case 4: DecapTile<4>::run(a, tl, lds, &wbits, lane, wave, kDecapWaves); break;
     default: break;
     }
 }
@@ -220,32 +231,41 @@ constexpr RhoTable kRho = make_rho();
 
 __device__ __forceinline__ uint64_t rotl64(uint64_t v, int n) { return n ? (v << n) | (v >> (64 - n)) : v; }
 
+// One round of Keccak-f[1600]; RC is iota's constant, a compile-time value per round.
+template <uint64_t RC>
+__device__ __forceinline__ void keccak_round(uint64_t (&A)[25])
+{
+    uint64_t C[5], B[25];
+#pragma unroll
+    for (int x = 0; x < 5; x++) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        const uint64_t D = C[(x + 4) % 5] ^ rotl64(C[(x + 1) % 5], 1);
+#pragma unroll
+        for (int y = 0; y < 5; y++) A[x + 5 * y] ^= D;
+    }
+    // rho and pi: lane (x, y) moves to (y, 2x + 3y)
+#pragma unroll
+    for (int x = 0; x < 5; x++)
+#pragma unroll
+        for (int y = 0; y < 5; y++) B[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(A[x + 5 * y], kRho.r[x + 5 * y]);
+    // chi
+#pragma unroll
+    for (int y = 0; y < 5; y++)
+#pragma unroll
+        for (int x = 0; x < 5; x++) A[x + 5 * y] = B[x + 5 * y] ^ (~B[(x + 1) % 5 + 5 * y] & B[(x + 2) % 5 + 5 * y]);
+    A[0] ^= RC;
+}
+
+template <int... Rs>
+__device__ __forceinline__ void keccak_rounds(uint64_t (&A)[25], std::integer_sequence<int, Rs...>)
+{
+    (keccak_round<round_constant(Rs)>(A), ...);
+}
+
 __device__ __forceinline__ void keccak_f1600(uint64_t (&A)[25])
 {
-#pragma unroll
-    for (int round = 0; round < 24; round++) {
-        uint64_t C[5], B[25];
-#pragma unroll
-        for (int x = 0; x < 5; x++) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
-#pragma unroll
-        for (int x = 0; x < 5; x++) {
-            const uint64_t D = C[(x + 4) % 5] ^ rotl64(C[(x + 1) % 5], 1);
-#pragma unroll
-            for (int y = 0; y < 5; y++) A[x + 5 * y] ^= D;
-        }
-        // rho and pi: lane (x, y) moves to (y, 2x + 3y)
-#pragma unroll
-        for (int x = 0; x < 5; x++)
-#pragma unroll
-            for (int y = 0; y < 5; y++) B[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(A[x + 5 * y], kRho.r[x + 5 * y]);
-        // chi
-#pragma unroll
-        for (int y = 0; y < 5; y++)
-#pragma unroll
-            for (int x = 0; x < 5; x++)
-                A[x + 5 * y] = B[x + 5 * y] ^ (~B[(x + 1) % 5 + 5 * y] & B[(x + 2) % 5 + 5 * y]);
-        A[0] ^= round_constant(round);
-    }
+    keccak_rounds(A, std::make_integer_sequence<int, 24>{});
 }
 
 struct KemSizes {
@@ -276,18 +296,24 @@ __global__ __launch_bounds__(64) void k_kem_pack(KemHashArgs a)
     const uint32_t b = a.ok[item] & 1u, keep = 0u - b;  // all ones when e is accepted
     const uint8_t *er = a.e + (size_t)item * a.e_stride;
     const uint8_t *cr = a.ct + (size_t)item * a.ct_stride;
-    uint8_t *dst = a.msg + (size_t)q * kKemMsgStride;
-    for (int o = lane; o < total; o += 64) {
-        uint32_t v = 0;
-        if (o == 0)
-            v = b;
-        else if (o <= z.eb)
-            v = ((uint32_t)er[o - 1] & keep) | ((uint32_t)kd.s[o - 1] & ~keep);
-        else if (o < len)
-            v = cr[o - 1 - z.eb];
-        if (o == len) v ^= 0x1Fu;
-        if (o == total - 1) v ^= 0x80u;
-        dst[o] = (uint8_t)v;
+    uint64_t *dst = reinterpret_cast<uint64_t *>(a.msg + (size_t)q * kKemMsgStride);
+    for (int w = lane; w < total / 8; w += 64) {
+        uint64_t word = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int o = 8 * w + j;
+            uint32_t v = 0;
+            if (o == 0)
+                v = b;
+            else if (o <= z.eb)
+                v = ((uint32_t)er[o - 1] & keep) | ((uint32_t)kd.s[o - 1] & ~keep);
+            else if (o < len)
+                v = cr[o - 1 - z.eb];
+            if (o == len) v ^= 0x1Fu;
+            if (o == total - 1) v ^= 0x80u;
+            word |= (uint64_t)(v & 0xFFu) << (8 * j);
+        }
+        dst[w] = word;
     }
 }
 
@@ -318,7 +344,7 @@ __global__ __launch_bounds__(256) void k_keccak(KemHashArgs a)
 hipError_t launch_decap_check(const DecapArgs &a, hipStream_t st)
 {
     if (a.n_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_decap_check, dim3(a.n_tiles), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_decap_check, dim3(a.n_tiles), dim3(64 * kDecapWaves), 0, st, a);
     return hipGetLastError();
 }
 
