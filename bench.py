@@ -6,7 +6,8 @@ assigned round-robin to the keys, each with an exactly-weight-128 random error. 
 is one decryption of the whole batch by the fused persistent kernel (one hipGraph launch),
 with ciphertexts already resident in HBM.  The plain per-stage pipeline (three launches per
 key group) runs the same batch for the fused/plain ratio; KEM decapsulation (decryption,
-re-encryption check and SHAKE256 session key) runs it as crypto_kem_dec; a mixed-parameter Poisson
+re-encryption check and SHAKE256 session key) runs it as crypto_kem_dec; host-buffer calls
+(four copies of the batch per call, PCIe included) time the chunked copy/kernel pipeline; a mixed-parameter Poisson
 request stream through the serving loop gives p50/p99 latency; the reference CPU decoder
 (built from its own C sources) is timed on a bounded sample on this host.
 
@@ -119,6 +120,35 @@ def kem_leg(cm, torch, ctx, key_ids, d_ct, ct, e_true, steps, warmup, stream):
                 for i in range(n))
     return {"value": round(n * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 4),
             "kernel_ms": round(kern_ms, 4), "bit_exact": bool(exact)}
+
+
+def host_io_leg(cm, ctx, key_ids, ct, e_true, reps, steps):
+    """PCIe-inclusive decryption from host buffers (cmd_decrypt): the headline batch repeated
+    `reps` times in one call, so the chunked pipeline overlaps copies and kernels; once with
+    pageable numpy buffers (staged) and once with cmd_host_alloc buffers (direct DMA)."""
+    n = len(ct) * reps
+    kids = np.tile(key_ids, reps)
+    big = np.tile(ct, (reps, 1))
+    want = np.tile(e_true, (reps, 1))
+    out = {"items": int(n), "chunk_items": 65536}
+
+    def run(src, dst):
+        ctx.decrypt(kids, src, out=dst)  # warm-up: slot buffers
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.decrypt(kids, src, out=dst)
+        wall = (time.perf_counter() - t0) / steps
+        return {"value": round(n / wall, 1), "ms_per_call": round(wall * 1e3, 3),
+                "bit_exact": bool(np.array_equal(dst, want))}
+
+    out["pageable"] = run(big, np.empty_like(want))
+    h_ct = ctx.host_array(big.shape)
+    h_e = ctx.host_array(want.shape)
+    h_ct[:] = big
+    out["pinned"] = run(h_ct, h_e)
+    ctx.host_free(h_ct)
+    ctx.host_free(h_e)
+    return out
 
 
 def key_pool(cm, ctx, n_keys, seed, threads):
@@ -360,6 +390,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-kem", action="store_true")
+    ap.add_argument("--no-host-io", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serve-keys", type=int, default=1024)
     ap.add_argument("--serve-rates", default="1e4,1e5,3e5,1e6")
@@ -431,6 +462,13 @@ def main():
         kem["decap_overhead_ms"] = round(kem["kernel_ms"] - kern_ms, 4)
         log(f"kem_dec: {kem['value']:,.0f} decaps/s, graph {kem['kernel_ms']:.3f} ms "
             f"(+{kem['decap_overhead_ms']:.3f} ms over decryption) exact={kem['bit_exact']}")
+
+    host_io = None
+    if not args.no_host_io:
+        host_io = host_io_leg(cm, ctx, key_ids, ct, e_true, 4, max(2, args.steps // 3))
+        log(f"host buffers, {host_io['items']} items per call: pageable {host_io['pageable']['value']:,.0f} dec/s, "
+            f"pinned {host_io['pinned']['value']:,.0f} dec/s "
+            f"exact={host_io['pageable']['bit_exact'] and host_io['pinned']['bit_exact']}")
 
     plain = None
     if not args.no_plain:
@@ -509,6 +547,7 @@ def main():
         "roofline_hbm": {"bound": "hbm", "achieved": round(hbm_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "kem_dec": kem,
+        "host_io_pcie_inclusive": host_io,
         "plain_6960119_4096": c2,
         "mixed_batch": mixed,
         "serve": serve,

@@ -142,6 +142,8 @@ _sig("cmd_batch_destroy", _I, _P)
 _sig("cmd_stage_synd", _I, _P, _I32, _SZ, _P, _SZ, _P)
 _sig("cmd_stage_synd_full", _I, _P, _I32, _SZ, _P, _SZ, _P)
 _sig("cmd_kem_dec", _I, _P, _SZ, _P, _P, _SZ, _P, _SZ)
+_sig("cmd_host_alloc", _I, _P, _SZ, _c.POINTER(_P))
+_sig("cmd_host_free", _I, _P, _P)
 _sig("cmd_stage_bm", _I, _P, _I32, _SZ, _P, _P)
 _sig("cmd_stage_root", _I, _P, _I32, _SZ, _P, _P, _SZ)
 _sig("cmd_schedule_get", _I, _P, _I32, _c.POINTER(Schedule))
@@ -350,10 +352,12 @@ class Context:
         self.key_param.pop(key_id, None)
 
     def decrypt(self, key_ids: Sequence[int], ct: np.ndarray, mode: int = MODE_FUSED,
-                ct_stride: Optional[int] = None, e_stride: Optional[int] = None) -> np.ndarray:
+                ct_stride: Optional[int] = None, e_stride: Optional[int] = None,
+                out: Optional[np.ndarray] = None) -> np.ndarray:
         """Error vectors for ciphertext rows `ct` (count x ct_stride), item i under key_ids[i].
 
-        Returns count x e_stride bytes (e_stride defaults to the largest n/8 of the keys)."""
+        Returns count x e_stride bytes (e_stride defaults to the largest n/8 of the keys), in
+        `out` when given (a C-contiguous uint8 array, e.g. from host_array)."""
         keys = np.ascontiguousarray(key_ids, dtype=np.int32)
         ct = np.ascontiguousarray(ct, dtype=np.uint8)
         count = keys.size
@@ -364,10 +368,27 @@ class Context:
             # itself rejects unknown keys with CMD_ERR_KEY
             e_stride = max((params(self.key_param[int(k)]).e_bytes if int(k) in self.key_param else 1024)
                            for k in np.unique(keys)) if count else 0
-        e = np.zeros((count, e_stride), np.uint8)
+        if out is None:
+            e = np.zeros((count, e_stride), np.uint8)
+        else:
+            if out.dtype != np.uint8 or not out.flags.c_contiguous or out.size < count * e_stride:
+                raise ValueError("out must be a C-contiguous uint8 array of count x e_stride bytes")
+            e = out
         _check(_lib.cmd_decrypt(self._h, mode, count, _ptr(keys), _ptr(ct), ct_stride, _ptr(e), e_stride),
                "cmd_decrypt")
         return e
+
+    def host_array(self, shape, dtype=np.uint8) -> np.ndarray:
+        """A numpy array in pinned host memory (cmd_host_alloc): decrypt() moves such buffers by
+        DMA without staging.  Valid until host_free() or until the context closes."""
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = _P()
+        _check(_lib.cmd_host_alloc(self._h, nbytes, _c.byref(p)), "cmd_host_alloc")
+        buf = (_c.c_uint8 * nbytes).from_address(p.value)
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def host_free(self, arr: np.ndarray):
+        _check(_lib.cmd_host_free(self._h, arr.ctypes.data), "cmd_host_free")
 
     def kem_dec(self, key_ids: Sequence[int], ct: np.ndarray, ct_stride: Optional[int] = None) -> np.ndarray:
         """Session keys (count x 32 bytes) of crypto_kem_dec for ciphertext rows `ct`."""

@@ -70,11 +70,20 @@ int main(int argc, char **argv)
         }
     }
 
-    std::vector<uint8_t> sk, ct;
+    std::vector<uint8_t> sk;
     const std::string p = std::to_string(param);
+    FILE *fct = nullptr;
     if (!read_file(vec_dir + "/sk_" + p + ".bin", sk, (size_t)info.sk_bytes) ||
-        !read_file(vec_dir + "/ct_" + p + ".bin", ct, (size_t)katnum * info.ct_bytes)) {
+        !(fct = std::fopen((vec_dir + "/ct_" + p + ".bin").c_str(), "rb"))) {
         std::perror("failed to read test vectors");
+        return CMD_ERR_FILE;
+    }
+    // a short ciphertext file fails before the device is touched, like the reference's fread
+    const size_t ct_bytes = (size_t)katnum * info.ct_bytes, e_bytes = (size_t)katnum * info.e_bytes;
+    if (std::fseek(fct, 0, SEEK_END) != 0 || std::ftell(fct) < (long)ct_bytes || std::fseek(fct, 0, SEEK_SET) != 0) {
+        std::fprintf(stderr, "failed to read test vectors: ct_%s.bin holds fewer than %ld ciphertexts\n", p.c_str(),
+                     katnum);
+        std::fclose(fct);
         return CMD_ERR_FILE;
     }
 
@@ -82,7 +91,27 @@ int main(int argc, char **argv)
     int rc = cmd_ctx_create(0, 0, &ctx);
     if (rc) {
         std::fprintf(stderr, "cmd_ctx_create: %s (%s)\n", cmd_strerror(rc), cmd_last_error());
+        std::fclose(fct);
         return rc;
+    }
+    // ciphertexts are read straight into pinned memory and the error vectors come back into
+    // pinned memory, so both cross PCIe by DMA
+    void *ct = nullptr, *e = nullptr;
+    rc = cmd_host_alloc(ctx, ct_bytes, &ct);
+    if (!rc) rc = cmd_host_alloc(ctx, e_bytes, &e);
+    if (rc) {
+        std::fprintf(stderr, "cmd_host_alloc: %s (%s)\n", cmd_strerror(rc), cmd_last_error());
+        std::fclose(fct);
+        cmd_ctx_destroy(ctx);
+        return rc;
+    }
+    const size_t got = std::fread(ct, 1, ct_bytes, fct);
+    std::fclose(fct);
+    if (got != ct_bytes) {
+        std::fprintf(stderr, "failed to read test vectors: %s holds fewer than %ld ciphertexts\n",
+                     (vec_dir + "/ct_" + p + ".bin").c_str(), katnum);
+        cmd_ctx_destroy(ctx);
+        return CMD_ERR_FILE;
     }
     int32_t key = -1;
     rc = cmd_key_load(ctx, param, sk.data(), sk.size(), &key);
@@ -92,17 +121,16 @@ int main(int argc, char **argv)
         return rc;
     }
     std::vector<int32_t> keys((size_t)katnum, key);
-    std::vector<uint8_t> e((size_t)katnum * info.e_bytes);
     auto t0 = std::chrono::steady_clock::now();
-    rc = cmd_decrypt(ctx, mode, (size_t)katnum, keys.data(), ct.data(), (size_t)info.ct_bytes, e.data(),
-                     (size_t)info.e_bytes);
+    rc = cmd_decrypt(ctx, mode, (size_t)katnum, keys.data(), (const uint8_t *)ct, (size_t)info.ct_bytes,
+                     (uint8_t *)e, (size_t)info.e_bytes);
     auto t1 = std::chrono::steady_clock::now();
     if (rc) {
         std::fprintf(stderr, "cmd_decrypt: %s (%s)\n", cmd_strerror(rc), cmd_last_error());
         cmd_ctx_destroy(ctx);
         return rc;
     }
-    rc = cmd_errorstream_write(param, out_path.c_str(), e.data(), (size_t)info.e_bytes, (size_t)katnum, 0);
+    rc = cmd_errorstream_write(param, out_path.c_str(), (const uint8_t *)e, (size_t)info.e_bytes, (size_t)katnum, 0);
     cmd_ctx_destroy(ctx);
     if (rc) {
         std::fprintf(stderr, "failed to write %s\n", out_path.c_str());

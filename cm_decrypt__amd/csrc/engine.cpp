@@ -272,59 +272,69 @@ int batch_enqueue(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
         }
         return CMD_OK;
     }
-    if (b->tiles.empty()) return CMD_OK;
+    return enqueue_fused(ctx, b->mode, b->sched, b->tiles, b->launch_tile, b->d_tiles, b->d_items, b->d_item_key,
+                         b->d_counter, ct, b->ct_stride, e, b->e_stride,
+                         KemScratch{(uint8_t *)b->d_kem_e.ptr, b->kem_e_stride, (uint32_t *)b->d_ok.ptr,
+                                    (uint8_t *)b->d_msg.ptr},
+                         st);
+}
+
+int enqueue_fused(cmd_ctx *ctx, int mode, const FusedSchedule sched[], const std::vector<FusedTile> &tiles,
+                  int launch_tile, const FusedTile *d_tiles, const int32_t *d_items, const int32_t *d_item_key,
+                  uint32_t *d_counter, const uint8_t *ct, size_t ct_stride, uint8_t *out, size_t out_stride,
+                  const KemScratch &kem, hipStream_t st)
+{
+    if (tiles.empty()) return CMD_OK;
+    const bool is_kem = mode == CMD_MODE_KEM;
+    size_t n_items = 0;
+    for (const FusedTile &t : tiles) n_items += (size_t)t.count;
     FusedArgs a;
-    a.tiles = b->d_tiles;
-    a.n_tiles = (int32_t)b->tiles.size();
+    a.tiles = d_tiles;
+    a.n_tiles = (int32_t)tiles.size();
     a.pad0 = 0;
-    a.items = b->d_items;
-    a.item_key = b->d_item_key;
+    a.items = d_items;
+    a.item_key = d_item_key;
     a.keys = ctx->d_keys;
     a.ct = ct;
-    a.e = e;
-    a.ct_stride = b->ct_stride;
-    a.e_stride = b->e_stride;
-    a.tile_counter = b->d_counter;
+    a.e = is_kem ? kem.e : out;
+    a.ct_stride = ct_stride;
+    a.e_stride = is_kem ? kem.e_stride : out_stride;
+    a.tile_counter = d_counter;
     a.fft = ctx->d_fft;
-    const bool kem = b->mode == CMD_MODE_KEM;
-    if (kem) {
-        a.e = (uint8_t *)b->d_kem_e.ptr;
-        a.e_stride = b->kem_e_stride;
-    }
-    const FusedSchedule s = launch_schedule(b->sched, b->tiles, b->launch_tile);
-    CMD_HIP(launch_zero_u32(b->d_counter, 1, st));
+    const FusedSchedule s = launch_schedule(sched, tiles, launch_tile);
+    CMD_HIP(launch_zero_u32(d_counter, 1, st));
     CMD_HIP(launch_fused(a, s, ctx->n_cu, st));
-    if (!kem) return CMD_OK;
+    if (!is_kem) return CMD_OK;
 
     DecapArgs d;
-    d.tiles = b->d_tiles;
-    d.n_tiles = (int32_t)b->tiles.size();
+    d.tiles = d_tiles;
+    d.n_tiles = (int32_t)tiles.size();
     d.pad0 = 0;
-    d.items = b->d_items;
-    d.item_key = b->d_item_key;
+    d.items = d_items;
+    d.item_key = d_item_key;
     d.keys = ctx->d_keys;
     d.ct = ct;
-    d.e = (const uint8_t *)b->d_kem_e.ptr;
-    d.ct_stride = b->ct_stride;
-    d.e_stride = b->kem_e_stride;
-    d.ok = (uint32_t *)b->d_ok.ptr;
+    d.e = kem.e;
+    d.ct_stride = ct_stride;
+    d.e_stride = kem.e_stride;
+    d.ok = kem.ok;
     d.s_out = nullptr;
     d.fft = ctx->d_fft;
     CMD_HIP(launch_decap_check(d, st));
     KemHashArgs h;
-    h.count = (int32_t)b->items.size();
+    h.count = (int32_t)n_items;
     h.pad0 = 0;
-    h.items = b->d_items;
-    h.item_key = b->d_item_key;
+    h.items = d_items;
+    h.item_key = d_item_key;
     h.keys = ctx->d_keys;
     h.ct = ct;
-    h.e = (const uint8_t *)b->d_kem_e.ptr;
-    h.ct_stride = b->ct_stride;
-    h.e_stride = b->kem_e_stride;
-    h.ok = (const uint32_t *)b->d_ok.ptr;
-    h.msg = (uint8_t *)b->d_msg.ptr;
-    h.key_out = e;
-    h.key_stride = b->e_stride;
+    h.e = kem.e;
+    h.ct_stride = ct_stride;
+    h.e_stride = kem.e_stride;
+    h.ok = kem.ok;
+    h.msg = kem.msg;
+    h.key_out = out;
+    h.key_stride = out_stride;
     CMD_HIP(launch_kem_hash(h, st));
     return CMD_OK;
 }
@@ -578,6 +588,7 @@ extern "C" int cmd_ctx_destroy(cmd_ctx *ctx)
     if (!ctx) return CMD_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    io_release(ctx);
     for (auto &k : ctx->keys)
         if (k.mem) (void)hipFree(k.mem);
     if (ctx->d_keys) (void)hipFree(ctx->d_keys);
@@ -838,6 +849,8 @@ extern "C" int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32
     if (!ct || !e_out || !key_ids) return CMD_ERR_ARG;
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     CMD_HIP(hipSetDevice(ctx->device));
+    if (mode == CMD_MODE_FUSED || mode == CMD_MODE_KEM)
+        return io_decrypt(ctx, mode, count, key_ids, ct, ct_stride, e_out, e_stride);
     int rc = dev_reserve(ctx, ctx->ct, count * ct_stride);
     if (!rc) rc = dev_reserve(ctx, ctx->e, count * e_stride);
     if (rc) return rc;

@@ -45,6 +45,20 @@ struct KeySlot {
 
 struct Engine;
 
+// One stage of the host-buffer pipeline (hostio.cpp): a stream, the device buffers of one
+// chunk and pinned staging.
+constexpr size_t kIoSlots = 3;
+struct IoSlot {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr, t0 = nullptr, t1 = nullptr;  // t0/t1 bracket the kernels when timing
+    uint8_t *h_ct = nullptr, *h_out = nullptr;
+    char *h_plan = nullptr;
+    size_t h_ct_cap = 0, h_out_cap = 0, h_plan_cap = 0;
+    DevBuf d_ct, d_out, d_plan, d_kem_e, d_ok, d_msg;
+    bool busy = false, staged_out = false, timed = false;
+    size_t first = 0, count = 0;
+};
+
 }  // namespace cmd
 
 // Opaque handles of the C ABI.
@@ -65,6 +79,9 @@ struct cmd_ctx {
 
     cmd::FusedSchedule sched[cmd::kNumParams];
     cmd::FftTab *d_fft = nullptr;   // kFftVariants additive-FFT constant tables
+
+    std::vector<cmd::IoSlot> io;                            // host-buffer pipeline stages
+    std::vector<std::pair<uintptr_t, size_t>> host_pinned;  // cmd_host_alloc blocks
 
     bool timing = false;
     double timing_us = 0.0;
@@ -126,6 +143,26 @@ FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<Fus
 int batch_plan(cmd_batch *b, const int32_t *key_ids);
 // Enqueues the batch's launches on `st` (no graph).
 int batch_enqueue(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st);
+
+// Host-buffer decryption (fused or KEM mode) through the chunked pipeline; io_release frees
+// its slots and the context's pinned blocks.
+int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, const uint8_t *ct, size_t ct_stride,
+               uint8_t *out, size_t out_stride);
+void io_release(cmd_ctx *ctx);
+
+// Device scratch of a KEM launch: decoded error vectors, accept flags, SHAKE256 inputs.
+struct KemScratch {
+    uint8_t *e;
+    size_t e_stride;
+    uint32_t *ok;
+    uint8_t *msg;
+};
+// Enqueues a planned fused launch (tiles already on the device, d_counter one word) on `st`;
+// for CMD_MODE_KEM also the check and the session-key kernels, `out` receiving the keys.
+int enqueue_fused(cmd_ctx *ctx, int mode, const FusedSchedule sched[], const std::vector<FusedTile> &tiles,
+                  int launch_tile, const FusedTile *d_tiles, const int32_t *d_items, const int32_t *d_item_key,
+                  uint32_t *d_counter, const uint8_t *ct, size_t ct_stride, uint8_t *out, size_t out_stride,
+                  const KemScratch &kem, hipStream_t st);
 void batch_release(cmd_batch *b);
 
 FusedSchedule default_schedule(int pidx);

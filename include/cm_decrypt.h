@@ -141,9 +141,19 @@ int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes);
 /* Batch decryption, host buffers (copied in and out, blocking).  Item i uses key key_ids[i]
  * and its ciphertext at ct + i*ct_stride; its error vector (n/8 bytes, the reference's
  * build_error_vector, decrypt.c:43-53) goes to e_out + i*e_stride, and the rest of each
- * e_stride row is zero.  Sets and keys may mix. */
+ * e_stride row is zero.  Sets and keys may mix.  Replaces the reference's chunked batch
+ * driver (GPU_Baseline/param/param8192128/src/cuda/Decrypt.cu:177-324): fused and KEM modes
+ * run chunks of up to 65536 items through a three-stage pipeline (copy in, kernels, copy
+ * out of different chunks overlap); buffers from cmd_host_alloc are copied by DMA
+ * directly, others through pinned staging. */
 int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
                 const uint8_t *ct, size_t ct_stride, uint8_t *e_out, size_t e_stride);
+
+/* Pinned host memory for cmd_decrypt / cmd_kem_dec buffers (ciphertexts read straight into
+ * it, results formatted straight from it), freed with cmd_host_free or with the context.
+ * No counterpart in the reference, which stages through pageable memory. */
+int cmd_host_alloc(cmd_ctx *ctx, size_t bytes, void **ptr);
+int cmd_host_free(cmd_ctx *ctx, void *ptr);
 
 /* Same on device buffers already resident in HBM, enqueued on `stream` (a hipStream_t,
  * NULL = the context's stream); key_ids stay on the host.  Only the n/8 bytes of each row
@@ -255,9 +265,9 @@ int cmd_server_reset_stats(cmd_server *s);
 int cmd_server_destroy(cmd_server *s);
 
 /* ---- measurement helpers -------------------------------------------------------------- */
-/* Average duration in microseconds of the dominant kernel of the last
- * cmd_batch_run/cmd_decrypt_device calls made with timing on (HIP events on the launch
- * stream), and resets the accumulator. */
+/* Average duration in microseconds of the kernels of the last cmd_batch_run /
+ * cmd_decrypt_device launches, and of each pipeline chunk of cmd_decrypt, made with timing
+ * on (HIP events on the launch stream), and resets the accumulator. */
 int cmd_set_kernel_timing(cmd_ctx *ctx, int32_t on);
 int cmd_kernel_timing(cmd_ctx *ctx, double *avg_us, uint64_t *launches);
 

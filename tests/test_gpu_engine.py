@@ -346,3 +346,37 @@ def test_cli_drop_in_matches_reference_errorstream(tmp_path, mode):
     assert r.returncode == 0, r.stderr.decode()
     out = (tmp_path / "CPU" / "results" / "output" / "errorstream0_348864.bin").read_bytes()
     assert out == gold["stream"].tobytes()
+
+
+def test_host_pipeline_chunks_pageable_and_pinned(ctx, keys):
+    """Host-buffer calls larger than one chunk run through the three-slot pipeline (five
+    chunks here, so slots are reused): results equal the encrypted errors for pageable and
+    for cmd_host_alloc buffers, in fused and KEM modes, and the rows' padding is zero."""
+    pid = 348864
+    sk, kid = keys[pid]
+    p = cm.params(pid)
+    ct1, e1 = cm.gen_encrypt(pid, sk, 71, 4099)
+    reps = 64  # 262336 items -> 5 chunks of about 52k
+    ct = np.tile(ct1, (reps, 1))
+    want = np.zeros((len(ct), 448), np.uint8)
+    want[:, :p.e_bytes] = np.tile(e1, (reps, 1))
+    print(f"host pipeline items={len(ct)}", flush=True)
+    kids = np.full(len(ct), kid, np.int32)
+    got = ctx.decrypt(kids, ct, e_stride=448)
+    assert np.array_equal(got, want)
+    h_ct = ctx.host_array(ct.shape)
+    h_e = ctx.host_array((len(ct), 448))
+    try:
+        h_ct[:] = ct
+        h_e[:] = 0xA5
+        ctx.decrypt(kids, h_ct, e_stride=448, out=h_e)
+        assert np.array_equal(h_e, want)
+        # KEM through the pipeline: every key is SHAKE256(1 || e || c)
+        k = ctx.kem_dec(kids[:70000], ct[:70000])
+        import hashlib
+        for i in (0, 1, 4098, 4099, 65535, 65536, 69999):
+            assert k[i].tobytes() == hashlib.shake_256(b"\x01" + want[i, :p.e_bytes].tobytes()
+                                                       + ct[i].tobytes()).digest(32)
+    finally:
+        ctx.host_free(h_ct)
+        ctx.host_free(h_e)
