@@ -33,6 +33,7 @@ MODE_PLAIN = 0
 MODE_FUSED = 1
 MODE_KEM = 2           # rows out are 32-byte session keys (crypto_kem_dec)
 KEM_KEY_BYTES = 32
+MODE_POSITIONS = 3     # rows out are t uint16 error positions, ascending, 0xFFFF-padded
 
 CMD_OK = 0
 CMD_ERR_FILE = -1
@@ -143,6 +144,7 @@ _sig("cmd_stage_synd", _I, _P, _I32, _SZ, _P, _SZ, _P)
 _sig("cmd_stage_synd_full", _I, _P, _I32, _SZ, _P, _SZ, _P)
 _sig("cmd_kem_dec", _I, _P, _SZ, _P, _P, _SZ, _P, _SZ)
 _sig("cmd_host_alloc", _I, _P, _SZ, _c.POINTER(_P))
+_sig("cmd_errorstream_format_positions", _I, _I32, _P, _SZ, _SZ, _P, _SZ, _c.POINTER(_SZ))
 _sig("cmd_host_free", _I, _P, _P)
 _sig("cmd_stage_bm", _I, _P, _I32, _SZ, _P, _P)
 _sig("cmd_stage_root", _I, _P, _I32, _SZ, _P, _P, _SZ)
@@ -243,6 +245,19 @@ def errorstream(param_id: int, e: np.ndarray) -> bytes:
     buf = _c.create_string_buffer(need.value)
     _check(_lib.cmd_errorstream_format(param_id, _ptr(e), p.e_bytes, e.shape[0], _c.addressof(buf), need.value,
                                        _c.byref(need)), "cmd_errorstream_format")
+    return buf.raw[:need.value]
+
+
+def errorstream_positions(param_id: int, pos: np.ndarray) -> bytes:
+    """The same text from position rows (count x t uint16, 0xFFFF-padded; CMD_MODE_POSITIONS)."""
+    p = params(param_id)
+    pos = np.ascontiguousarray(pos, dtype=np.uint16).reshape(-1, p.t)
+    need = _SZ(0)
+    _check(_lib.cmd_errorstream_format_positions(param_id, _ptr(pos), 2 * p.t, pos.shape[0], None, 0,
+                                                 _c.byref(need)), "cmd_errorstream_format_positions")
+    buf = _c.create_string_buffer(max(1, need.value))
+    _check(_lib.cmd_errorstream_format_positions(param_id, _ptr(pos), 2 * p.t, pos.shape[0], _c.addressof(buf),
+                                                 need.value, _c.byref(need)), "cmd_errorstream_format_positions")
     return buf.raw[:need.value]
 
 
@@ -377,6 +392,14 @@ class Context:
         _check(_lib.cmd_decrypt(self._h, mode, count, _ptr(keys), _ptr(ct), ct_stride, _ptr(e), e_stride),
                "cmd_decrypt")
         return e
+
+    def positions(self, key_ids: Sequence[int], ct: np.ndarray, ct_stride: Optional[int] = None) -> np.ndarray:
+        """Error positions (count x t_max uint16, ascending, 0xFFFF-padded) of CMD_MODE_POSITIONS."""
+        keys = np.ascontiguousarray(key_ids, dtype=np.int32)
+        t = max((params(self.key_param[int(k)]).t if int(k) in self.key_param else 128)
+                for k in np.unique(keys)) if keys.size else 1
+        rows = self.decrypt(keys, ct, mode=MODE_POSITIONS, ct_stride=ct_stride, e_stride=2 * t)
+        return rows.view(np.uint16)
 
     def host_array(self, shape, dtype=np.uint8) -> np.ndarray:
         """A numpy array in pinned host memory (cmd_host_alloc): decrypt() moves such buffers by

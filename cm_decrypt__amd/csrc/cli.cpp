@@ -79,7 +79,10 @@ int main(int argc, char **argv)
         return CMD_ERR_FILE;
     }
     // a short ciphertext file fails before the device is touched, like the reference's fread
-    const size_t ct_bytes = (size_t)katnum * info.ct_bytes, e_bytes = (size_t)katnum * info.e_bytes;
+    // the fused path brings back error positions (2t bytes per ciphertext) instead of the
+    // n/8-byte bitmaps; the plain path keeps the bitmaps of the per-stage kernels
+    const size_t row = mode == CMD_MODE_PLAIN ? (size_t)info.e_bytes : (size_t)2 * info.t;
+    const size_t ct_bytes = (size_t)katnum * info.ct_bytes, e_bytes = (size_t)katnum * row;
     if (std::fseek(fct, 0, SEEK_END) != 0 || std::ftell(fct) < (long)ct_bytes || std::fseek(fct, 0, SEEK_SET) != 0) {
         std::fprintf(stderr, "failed to read test vectors: ct_%s.bin holds fewer than %ld ciphertexts\n", p.c_str(),
                      katnum);
@@ -122,15 +125,17 @@ int main(int argc, char **argv)
     }
     std::vector<int32_t> keys((size_t)katnum, key);
     auto t0 = std::chrono::steady_clock::now();
-    rc = cmd_decrypt(ctx, mode, (size_t)katnum, keys.data(), (const uint8_t *)ct, (size_t)info.ct_bytes,
-                     (uint8_t *)e, (size_t)info.e_bytes);
+    rc = cmd_decrypt(ctx, mode == CMD_MODE_PLAIN ? CMD_MODE_PLAIN : CMD_MODE_POSITIONS, (size_t)katnum, keys.data(),
+                     (const uint8_t *)ct, (size_t)info.ct_bytes, (uint8_t *)e, row);
     auto t1 = std::chrono::steady_clock::now();
     if (rc) {
         std::fprintf(stderr, "cmd_decrypt: %s (%s)\n", cmd_strerror(rc), cmd_last_error());
         cmd_ctx_destroy(ctx);
         return rc;
     }
-    rc = cmd_errorstream_write(param, out_path.c_str(), (const uint8_t *)e, (size_t)info.e_bytes, (size_t)katnum, 0);
+    rc = mode == CMD_MODE_PLAIN
+             ? cmd_errorstream_write(param, out_path.c_str(), (const uint8_t *)e, row, (size_t)katnum, 0)
+             : cmd_errorstream_write_positions(param, out_path.c_str(), (const uint16_t *)e, row, (size_t)katnum, 0);
     cmd_ctx_destroy(ctx);
     if (rc) {
         std::fprintf(stderr, "failed to write %s\n", out_path.c_str());

@@ -167,6 +167,19 @@ FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<Fus
     return s;
 }
 
+size_t out_row_bytes(int mode, size_t count, const int32_t *key_ids, const cmd_ctx *ctx)
+{
+    size_t e = 0, t = 0;
+    for (size_t i = 0; i < count; i++) {
+        const ParamDesc &p = kParams[ctx->keys[key_ids[i]].pidx];
+        e = std::max(e, (size_t)p.n / 8);
+        t = std::max(t, (size_t)p.t);
+    }
+    if (mode == CMD_MODE_KEM) return CMD_KEM_KEY_BYTES;
+    if (mode == CMD_MODE_POSITIONS) return 2 * t;
+    return e;
+}
+
 int batch_plan(cmd_batch *b, const int32_t *key_ids)
 {
     cmd_ctx *ctx = b->ctx;
@@ -181,10 +194,12 @@ int batch_plan(cmd_batch *b, const int32_t *key_ids)
         max_ct = std::max(max_ct, (size_t)p.synd_bytes);
         max_e = std::max(max_e, (size_t)p.n / 8);
     }
-    const size_t max_out = b->mode == CMD_MODE_KEM ? (size_t)CMD_KEM_KEY_BYTES : max_e;
+    const size_t max_out = out_row_bytes(b->mode, count, key_ids, ctx);
     if (b->ct_stride < max_ct || b->e_stride < max_out)
         return set_error(CMD_ERR_ARG, "strides (%zu, %zu) below item sizes (%zu, %zu)", b->ct_stride, b->e_stride,
                          max_ct, max_out);
+    if (b->mode == CMD_MODE_POSITIONS && (b->e_stride & 1u))
+        return set_error(CMD_ERR_ARG, "position rows need an even stride (%zu)", b->e_stride);
 
     // order items by (set, key), keeping input order inside a key
     std::vector<int32_t> order(count);
@@ -212,7 +227,7 @@ int batch_plan(cmd_batch *b, const int32_t *key_ids)
         i = j;
     }
 
-    if (b->mode == CMD_MODE_FUSED || b->mode == CMD_MODE_KEM) {
+    if (b->mode != CMD_MODE_PLAIN) {
         int rc = plan_fused(ctx, b->sched, count, key_ids, b->tiles, b->items, b->item_key, &b->launch_tile);
         if (rc) return rc;
     } else {
@@ -245,12 +260,12 @@ int batch_plan(cmd_batch *b, const int32_t *key_ids)
         if (!rc) rc = dev_alloc(ctx, b->d_loc, (size_t)b->max_group * (b->max_t + 1) * 2);
         if (rc) return rc;
     }
-    if (b->mode == CMD_MODE_KEM && count > 0) {
-        // decoded error vectors, accept flags and SHAKE256 inputs stay on the device
+    if ((b->mode == CMD_MODE_KEM || b->mode == CMD_MODE_POSITIONS) && count > 0) {
+        // decoded error vectors (and for KEM accept flags and SHAKE256 inputs) stay on the device
         b->kem_e_stride = (max_e + 3) / 4 * 4;
         rc = dev_alloc(ctx, b->d_kem_e, count * b->kem_e_stride);
-        if (!rc) rc = dev_alloc(ctx, b->d_ok, count * 4);
-        if (!rc) rc = dev_alloc(ctx, b->d_msg, count * (size_t)kKemMsgStride);
+        if (!rc && b->mode == CMD_MODE_KEM) rc = dev_alloc(ctx, b->d_ok, count * 4);
+        if (!rc && b->mode == CMD_MODE_KEM) rc = dev_alloc(ctx, b->d_msg, count * (size_t)kKemMsgStride);
         if (rc) return rc;
     }
     return CMD_OK;
@@ -285,7 +300,7 @@ int enqueue_fused(cmd_ctx *ctx, int mode, const FusedSchedule sched[], const std
                   const KemScratch &kem, hipStream_t st)
 {
     if (tiles.empty()) return CMD_OK;
-    const bool is_kem = mode == CMD_MODE_KEM;
+    const bool is_kem = mode == CMD_MODE_KEM, to_scratch = is_kem || mode == CMD_MODE_POSITIONS;
     size_t n_items = 0;
     for (const FusedTile &t : tiles) n_items += (size_t)t.count;
     FusedArgs a;
@@ -296,14 +311,27 @@ int enqueue_fused(cmd_ctx *ctx, int mode, const FusedSchedule sched[], const std
     a.item_key = d_item_key;
     a.keys = ctx->d_keys;
     a.ct = ct;
-    a.e = is_kem ? kem.e : out;
+    a.e = to_scratch ? kem.e : out;
     a.ct_stride = ct_stride;
-    a.e_stride = is_kem ? kem.e_stride : out_stride;
+    a.e_stride = to_scratch ? kem.e_stride : out_stride;
     a.tile_counter = d_counter;
     a.fft = ctx->d_fft;
     const FusedSchedule s = launch_schedule(sched, tiles, launch_tile);
     CMD_HIP(launch_zero_u32(d_counter, 1, st));
     CMD_HIP(launch_fused(a, s, ctx->n_cu, st));
+    if (mode == CMD_MODE_POSITIONS) {
+        PositionsArgs pa;
+        pa.count = (int32_t)n_items;
+        pa.pad0 = 0;
+        pa.items = d_items;
+        pa.item_key = d_item_key;
+        pa.keys = ctx->d_keys;
+        pa.e = kem.e;
+        pa.e_stride = kem.e_stride;
+        pa.out = out;
+        pa.out_stride = out_stride;
+        CMD_HIP(launch_positions(pa, st));
+    }
     if (!is_kem) return CMD_OK;
 
     DecapArgs d;
@@ -496,18 +524,13 @@ void parallel_chunks(size_t count, F fn)
     for (auto &th : pool) th.join();
 }
 
-}  // namespace
-
-extern "C" int cmd_errorstream_format(int32_t param_id, const uint8_t *e, size_t e_stride, size_t count, char *buf,
-                                      size_t cap, size_t *written)
+// Lines k < count measured by bytes_of(k) and written by write_line(k, dst) in parallel chunks.
+template <typename Bytes, typename Write>
+int format_lines(size_t count, Bytes bytes_of, Write write_line, char *buf, size_t cap, size_t *written)
 {
-    int pidx = param_index(param_id);
-    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
-    const int nbytes = kParams[pidx].n / 8;
-    if ((!e && count) || e_stride < (size_t)nbytes) return CMD_ERR_ARG;
     std::vector<size_t> off(count + 1, 0);
     parallel_chunks(count, [&](size_t a, size_t b) {
-        for (size_t k = a; k < b; k++) off[k + 1] = line_bytes(e + k * e_stride, nbytes);
+        for (size_t k = a; k < b; k++) off[k + 1] = bytes_of(k);
     });
     for (size_t k = 0; k < count; k++) off[k + 1] += off[k];
     const size_t total = off[count];
@@ -519,17 +542,75 @@ extern "C" int cmd_errorstream_format(int32_t param_id, const uint8_t *e, size_t
     // written with exact-size copies into a bounce buffer
     parallel_chunks(count, [&](size_t a, size_t b) {
         for (size_t k = a; k < b; k++) {
-            const uint8_t *v = e + k * e_stride;
             const size_t len = off[k + 1] - off[k];
             if (k + 1 < b && off[k + 1] + 3 < total) {
-                line_write(v, nbytes, buf + off[k]);
+                write_line(k, buf + off[k]);
             } else {
                 std::vector<char> tmp(len + 4);
-                line_write(v, nbytes, tmp.data());
+                write_line(k, tmp.data());
                 std::memcpy(buf + off[k], tmp.data(), len);
             }
         }
     });
+    return CMD_OK;
+}
+
+// Position rows: t entries, ascending, 0xFFFF (or any value >= n) ends the row.
+size_t pos_line_bytes(const uint16_t *pos, int t, int n)
+{
+    const DigitTable &d = digits();
+    size_t len = 1;
+    for (int i = 0; i < t && pos[i] < n; i++) len += 1 + d.len[pos[i]];
+    return len;
+}
+
+size_t pos_line_write(const uint16_t *pos, int t, int n, char *out)
+{
+    const DigitTable &d = digits();
+    char *p = out;
+    for (int i = 0; i < t && pos[i] < n; i++) {
+        *p++ = ' ';
+        std::memcpy(p, d.text[pos[i]], 4);
+        p += d.len[pos[i]];
+    }
+    *p++ = '\n';
+    return (size_t)(p - out);
+}
+
+}  // namespace
+
+extern "C" int cmd_errorstream_format(int32_t param_id, const uint8_t *e, size_t e_stride, size_t count, char *buf,
+                                      size_t cap, size_t *written)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    const int nbytes = kParams[pidx].n / 8;
+    if ((!e && count) || e_stride < (size_t)nbytes) return CMD_ERR_ARG;
+    return format_lines(
+        count, [&](size_t k) { return line_bytes(e + k * e_stride, nbytes); },
+        [&](size_t k, char *dst) { line_write(e + k * e_stride, nbytes, dst); }, buf, cap, written);
+}
+
+extern "C" int cmd_errorstream_format_positions(int32_t param_id, const uint16_t *pos, size_t pos_stride,
+                                                size_t count, char *buf, size_t cap, size_t *written)
+{
+    int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    const int t = kParams[pidx].t, n = kParams[pidx].n;
+    if ((!pos && count) || pos_stride < (size_t)(2 * t) || (pos_stride & 1u)) return CMD_ERR_ARG;
+    const size_t st = pos_stride / 2;
+    return format_lines(
+        count, [&](size_t k) { return pos_line_bytes(pos + k * st, t, n); },
+        [&](size_t k, char *dst) { pos_line_write(pos + k * st, t, n, dst); }, buf, cap, written);
+}
+
+static int write_text(const char *path, const std::vector<char> &buf, int append)
+{
+    FILE *f = std::fopen(path, append ? "ab" : "wb");
+    if (!f) return set_error(CMD_ERR_FILE, "cannot open %s", path);
+    size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+    int cl = std::fclose(f);
+    if (w != buf.size() || cl != 0) return set_error(CMD_ERR_FILE, "short write to %s", path);
     return CMD_OK;
 }
 
@@ -541,13 +622,18 @@ extern "C" int cmd_errorstream_write(int32_t param_id, const char *path, const u
     if (rc) return rc;
     std::vector<char> buf(need);
     if (need) rc = cmd_errorstream_format(param_id, e, e_stride, count, buf.data(), buf.size(), &need);
+    return rc ? rc : write_text(path, buf, append);
+}
+
+extern "C" int cmd_errorstream_write_positions(int32_t param_id, const char *path, const uint16_t *pos,
+                                               size_t pos_stride, size_t count, int append)
+{
+    size_t need = 0;
+    int rc = cmd_errorstream_format_positions(param_id, pos, pos_stride, count, nullptr, 0, &need);
     if (rc) return rc;
-    FILE *f = std::fopen(path, append ? "ab" : "wb");
-    if (!f) return set_error(CMD_ERR_FILE, "cannot open %s", path);
-    size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
-    int cl = std::fclose(f);
-    if (w != buf.size() || cl != 0) return set_error(CMD_ERR_FILE, "short write to %s", path);
-    return CMD_OK;
+    std::vector<char> buf(need);
+    if (need) rc = cmd_errorstream_format_positions(param_id, pos, pos_stride, count, buf.data(), buf.size(), &need);
+    return rc ? rc : write_text(path, buf, append);
 }
 
 extern "C" int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx **out)
@@ -724,7 +810,7 @@ extern "C" int cmd_key_unload(cmd_ctx *ctx, int32_t key_id)
 static int make_batch(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids, size_t ct_stride,
                       size_t e_stride, int32_t use_graph, cmd_batch **out)
 {
-    if (mode != CMD_MODE_PLAIN && mode != CMD_MODE_FUSED && mode != CMD_MODE_KEM)
+    if (mode != CMD_MODE_PLAIN && mode != CMD_MODE_FUSED && mode != CMD_MODE_KEM && mode != CMD_MODE_POSITIONS)
         return set_error(CMD_ERR_ARG, "mode %d", (int)mode);
     if (count && !key_ids) return CMD_ERR_ARG;
     if (count > (size_t)INT32_MAX) return set_error(CMD_ERR_ARG, "count %zu too large", count);
@@ -849,8 +935,9 @@ extern "C" int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32
     if (!ct || !e_out || !key_ids) return CMD_ERR_ARG;
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     CMD_HIP(hipSetDevice(ctx->device));
-    if (mode == CMD_MODE_FUSED || mode == CMD_MODE_KEM)
+    if (mode == CMD_MODE_FUSED || mode == CMD_MODE_KEM || mode == CMD_MODE_POSITIONS)
         return io_decrypt(ctx, mode, count, key_ids, ct, ct_stride, e_out, e_stride);
+    if (mode != CMD_MODE_PLAIN) return set_error(CMD_ERR_ARG, "mode %d", (int)mode);
     int rc = dev_reserve(ctx, ctx->ct, count * ct_stride);
     if (!rc) rc = dev_reserve(ctx, ctx->e, count * e_stride);
     if (rc) return rc;

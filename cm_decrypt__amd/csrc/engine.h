@@ -139,6 +139,8 @@ int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, co
 // Schedule a fused launch uses: tile width of the batch, waves/groups of its heaviest set.
 FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<FusedTile> &tiles, int launch_tile);
 
+// Output row bytes a mode needs for these items (keys already validated).
+size_t out_row_bytes(int mode, size_t count, const int32_t *key_ids, const cmd_ctx *ctx);
 // Builds the plan of a batch (host arrays) and uploads it.
 int batch_plan(cmd_batch *b, const int32_t *key_ids);
 // Enqueues the batch's launches on `st` (no graph).

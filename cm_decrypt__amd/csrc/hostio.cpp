@@ -1,6 +1,6 @@
 // Host-buffer decryption pipeline and pinned host memory.
 //
-// cmd_decrypt on host buffers (fused and KEM modes) cuts the items into chunks that flow
+// cmd_decrypt on host buffers (fused, KEM and positions modes) cuts the items into chunks that flow
 // through kIoSlots slots.  Each slot owns a stream, device buffers for one chunk and pinned
 // staging; a chunk's work list, ciphertexts, kernels and results are all enqueued on the
 // slot's stream, so the copy in of one chunk, the kernels of another and the copy out of a
@@ -115,7 +115,9 @@ int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, con
         max_ct = std::max(max_ct, (size_t)p.synd_bytes);
         max_e = std::max(max_e, (size_t)p.n / 8);
     }
-    const size_t need_out = mode == CMD_MODE_KEM ? (size_t)CMD_KEM_KEY_BYTES : max_e;
+    const size_t need_out = out_row_bytes(mode, count, key_ids, ctx);
+    if (mode == CMD_MODE_POSITIONS && (out_stride & 1u))
+        return set_error(CMD_ERR_ARG, "position rows need an even stride (%zu)", out_stride);
     if (ct_stride < max_ct || out_stride < need_out)
         return set_error(CMD_ERR_ARG, "strides (%zu, %zu) below item sizes (%zu, %zu)", ct_stride, out_stride, max_ct,
                          need_out);
@@ -159,9 +161,9 @@ int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, con
         if (!rc) rc = dev_reserve(ctx, s.d_plan, plan_bytes);
         if (!rc) rc = dev_reserve(ctx, s.d_ct, n * ct_stride);
         if (!rc) rc = dev_reserve(ctx, s.d_out, n * out_stride);
+        if (!rc && (mode == CMD_MODE_KEM || mode == CMD_MODE_POSITIONS)) rc = dev_reserve(ctx, s.d_kem_e, n * kem_e_stride);
         if (!rc && mode == CMD_MODE_KEM) {
-            rc = dev_reserve(ctx, s.d_kem_e, n * kem_e_stride);
-            if (!rc) rc = dev_reserve(ctx, s.d_ok, n * 4);
+            rc = dev_reserve(ctx, s.d_ok, n * 4);
             if (!rc) rc = dev_reserve(ctx, s.d_msg, n * (size_t)kKemMsgStride);
         }
         if (rc) break;

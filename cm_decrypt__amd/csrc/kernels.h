@@ -117,6 +117,23 @@ struct KemHashArgs {
 };
 hipError_t launch_kem_hash(const KemHashArgs &a, hipStream_t st);
 
+// ---- error positions -------------------------------------------------------------------
+// Slot q < count: the set bits of item items[q]'s decoded vector (e + item * e_stride, a
+// 4-byte-aligned row of n/8 bytes) as t ascending uint16 at out + item * out_stride, padded
+// with 0xFFFF.
+struct PositionsArgs {
+    int32_t count;
+    int32_t pad0;
+    const int32_t *items;
+    const int32_t *item_key;
+    const KeyDev *keys;
+    const uint8_t *e;
+    uint64_t e_stride;
+    uint8_t *out;
+    uint64_t out_stride;
+};
+hipError_t launch_positions(const PositionsArgs &a, hipStream_t st);
+
 // Host: fills the FFT constants for (m, l).
 void build_fft_tab(int m, int l, FftTab *out);
 

@@ -77,11 +77,17 @@ int cmd_key_expand_host(int32_t param_id, const uint8_t *sk, size_t sk_len,
  * Writes at most cap bytes to buf; *written receives the full length (call with cap 0 to size). */
 int cmd_errorstream_format(int32_t param_id, const uint8_t *e, size_t e_stride, size_t count,
                            char *buf, size_t cap, size_t *written);
+/* The same text from CMD_MODE_POSITIONS rows (t uint16 each, pos_stride bytes apart; 0xFFFF
+ * ends a row early). */
+int cmd_errorstream_format_positions(int32_t param_id, const uint16_t *pos, size_t pos_stride, size_t count,
+                                     char *buf, size_t cap, size_t *written);
 
 /* Writes (append != 0: appends) the errorstream file; the reference drivers reset it first
  * (decrypt.c:138) or append per ciphertext (CPU/param/param8192128/src/Decrypt.c:161). */
 int cmd_errorstream_write(int32_t param_id, const char *path, const uint8_t *e, size_t e_stride,
                           size_t count, int append);
+int cmd_errorstream_write_positions(int32_t param_id, const char *path, const uint16_t *pos,
+                                    size_t pos_stride, size_t count, int append);
 
 /* ---- seeded test-vector generator (replaces kem/: keypair, gen_e, syndrome) --------------
  * kem/common/core/operations.c:36-107 and encrypt.c:40-120 define the formats; the
@@ -115,6 +121,9 @@ typedef struct cmd_ctx cmd_ctx;
 #define CMD_MODE_KEM   2   /* fused decryption + decapsulation check + SHAKE256: the output row
                               of an item is its 32-byte session key (crypto_kem_dec) */
 #define CMD_KEM_KEY_BYTES 32
+#define CMD_MODE_POSITIONS 3  /* fused decryption; the output row of an item is its error
+                                 positions: t uint16 ascending, padded with 0xFFFF (2t bytes,
+                                 the errorstream's numbers without the n/8-byte bitmap) */
 
 /* device: HIP ordinal; hbm_budget_bytes: 0 = 90% of the free device memory. */
 int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx **out);

@@ -380,3 +380,46 @@ def test_host_pipeline_chunks_pageable_and_pinned(ctx, keys):
     finally:
         ctx.host_free(h_ct)
         ctx.host_free(h_e)
+
+
+def _positions_rows(e, n, t):
+    rows = np.full((len(e), t), 0xFFFF, np.uint16)
+    for i, v in enumerate(e):
+        p = np.flatnonzero(np.unpackbits(v, bitorder="little")[:n])[:t]
+        rows[i, :len(p)] = p
+    return rows
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_positions_mode_vs_oracle(ctx, keys, pid):
+    """CMD_MODE_POSITIONS rows equal the oracle's error vectors as positions: weight t,
+    weight < t (the L_i = 0 quirk may add one) and random ciphertexts."""
+    sk, kid = keys[pid]
+    p = cm.params(pid)
+    ct, _ = cm.gen_encrypt(pid, sk, 81, 40)
+    low, _ = cm.gen_encrypt(pid, sk, 82, 5, weight=p.t - 7)
+    rnd = np.random.default_rng(pid % 113).integers(0, 256, (8, p.ct_bytes), dtype=np.uint8)
+    cts = np.vstack([ct, low, rnd])
+    want = _positions_rows(orc.decrypt(pid, sk, cts), p.n, p.t)
+    print(f"positions pid={pid} items={len(cts)}", flush=True)
+    assert np.array_equal(ctx.positions([kid] * len(cts), cts), want)
+
+
+def test_positions_mixed_batch_graph(ctx, keys):
+    ct, kids, want_e = mixed_rows(keys, 19, 63)
+    pid_of = {kid: pid for pid, (_, kid) in keys.items()}
+    d_ct = torch.from_numpy(ct).cuda()
+    d_p = torch.zeros((len(ct), 256), dtype=torch.uint8, device="cuda")
+    b = cm.Batch(ctx, kids, 208, 256, mode=cm.MODE_POSITIONS, use_graph=True)
+    try:
+        for rep in range(3):
+            d_p.zero_()
+            b.run(d_ct.data_ptr(), d_p.data_ptr(), torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            got = d_p.cpu().numpy().view(np.uint16)
+            for i in range(len(ct)):
+                p = cm.params(pid_of[int(kids[i])])
+                w = _positions_rows(want_e[i:i + 1, :p.e_bytes], p.n, p.t)[0]
+                assert np.array_equal(got[i, :p.t], w), f"replay {rep} row {i}"
+    finally:
+        b.close()

@@ -185,3 +185,31 @@ def test_errorstream_parallel_formatter_vs_oracle(pid):
     for i in range(0, len(e), 13):
         assert lines[i] + b"\n" == orc.errorstream_line(pid, e[i]), i
     assert cm.errorstream(pid, e[:100]) == b"".join(orc.errorstream_line(pid, x) for x in e[:100])
+
+
+def _positions_rows(e, n, t):
+    rows = np.full((len(e), t), 0xFFFF, np.uint16)
+    for i, v in enumerate(e):
+        p = np.flatnonzero(np.unpackbits(v, bitorder="little")[:n])[:t]
+        rows[i, :len(p)] = p
+    return rows
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_errorstream_from_positions_matches_bitmaps(pid):
+    """Position rows (CMD_MODE_POSITIONS) format to the same text as the bitmaps: weight t,
+    weight < t, empty rows, the first and last positions, rows without padding."""
+    p = cm.params(pid)
+    rng = np.random.default_rng(pid % 997)
+    e = np.zeros((3000, p.e_bytes), np.uint8)
+    for i in range(len(e)):
+        w = [p.t, p.t - 9, 0, 1, p.t][i % 5]
+        pos = rng.choice(p.n, w, replace=False)
+        if i % 7 == 0 and w:
+            pos[0] = 0
+        if i % 11 == 0 and w:
+            pos[-1] = p.n - 1
+        for q in pos:
+            e[i, q >> 3] |= 1 << (q & 7)
+    rows = _positions_rows(e, p.n, p.t)
+    assert cm.errorstream_positions(pid, rows) == cm.errorstream(pid, e)
