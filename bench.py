@@ -50,8 +50,10 @@ def gen_workload(cm, pid, n_keys, n_ct, seed, threads):
     def enc(k):
         return cm.gen_encrypt(pid, sks[k], seed * 7919 + k, per[k])
 
+    t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(threads) as ex:
         parts = list(ex.map(enc, range(n_keys)))
+    gen_workload.encrypt_s = time.perf_counter() - t0
     p = cm.params(pid)
     ct = np.zeros((n_ct, p.ct_bytes), np.uint8)
     e = np.zeros((n_ct, p.e_bytes), np.uint8)
@@ -120,6 +122,23 @@ def kem_leg(cm, torch, ctx, key_ids, d_ct, ct, e_true, steps, warmup, stream):
                 for i in range(n))
     return {"value": round(n * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 4),
             "kernel_ms": round(kern_ms, 4), "bit_exact": bool(exact)}
+
+
+def generator_leg(cm, ctx, kids, ct, e_true, key_of, seed, cpu_s, threads):
+    """The workload's ciphertexts again from the GPU generator (per key: parity-check columns,
+    Gauss-Jordan inverse, then one wavefront per ciphertext) against the host generator's
+    time on `threads` threads; the bytes must be identical."""
+    n_keys = len(kids)
+    per = int(np.bincount(key_of).max())
+    seeds = [seed * 7919 + k for k in range(n_keys)]
+    ctx.gen_encrypt(kids[:1], seeds[:1], 1)  # warm-up
+    t0 = time.perf_counter()
+    g_ct, g_e = ctx.gen_encrypt(kids, seeds, per)
+    gpu_s = time.perf_counter() - t0
+    same = all(np.array_equal(g_ct[k * per:(k + 1) * per], ct[key_of == k]) and
+               np.array_equal(g_e[k * per:(k + 1) * per], e_true[key_of == k]) for k in range(n_keys))
+    return {"items": int(len(ct)), "keys": n_keys, "host_s": round(cpu_s, 3), "host_threads": threads,
+            "gpu_s": round(gpu_s, 3), "identical": bool(same)}
 
 
 def host_io_leg(cm, ctx, key_ids, ct, e_true, reps, steps):
@@ -239,13 +258,10 @@ def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads, inflight):
     templates = []  # (key_id, pid, ct, e)
     t0 = time.perf_counter()
     for pid, keys in pool.items():
-        def enc(k):
-            return cm.gen_encrypt(pid, keys[k][1], seed + 31 * k, 4)
-
-        with cf.ThreadPoolExecutor(threads) as ex:
-            encs = list(ex.map(enc, range(len(keys))))
+        # the GPU generator makes the host generator's ciphertexts (4 per key)
+        g_ct, g_e = ctx.gen_encrypt([kid for kid, _ in keys], [seed + 31 * k for k in range(len(keys))], 4)
         for k, (kid, _) in enumerate(keys):
-            for c, x in zip(*encs[k]):
+            for c, x in zip(g_ct[4 * k:4 * k + 4], g_e[4 * k:4 * k + 4]):
                 templates.append((kid, pid, c, x))
     pool = templates
     log(f"serving: {len(pool)} request templates over {sum(per_set.values())} resident keys "
@@ -407,6 +423,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-kem", action="store_true")
     ap.add_argument("--no-host-io", action="store_true")
+    ap.add_argument("--no-generator", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serve-keys", type=int, default=1024)
     ap.add_argument("--serve-rates", default="1e4,1e5,3e5,1e6")
@@ -478,6 +495,12 @@ def main():
         kem["decap_overhead_ms"] = round(kem["kernel_ms"] - kern_ms, 4)
         log(f"kem_dec: {kem['value']:,.0f} decaps/s, graph {kem['kernel_ms']:.3f} ms "
             f"(+{kem['decap_overhead_ms']:.3f} ms over decryption) exact={kem['bit_exact']}")
+
+    generator = None
+    if not args.no_generator:
+        generator = generator_leg(cm, ctx, kids, ct, e_true, key_of, 1, gen_workload.encrypt_s, threads)
+        log(f"generator, {generator['items']} ciphertexts of {generator['keys']} keys: host {generator['host_s']:.2f} s "
+            f"({threads} threads), GPU {generator['gpu_s']:.3f} s, identical={generator['identical']}")
 
     host_io = None
     if not args.no_host_io:
@@ -565,6 +588,7 @@ def main():
                          "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "kem_dec": kem,
         "host_io_pcie_inclusive": host_io,
+        "generator": generator,
         "plain_6960119_4096": c2,
         "mixed_batch": mixed,
         "serve": serve,

@@ -144,6 +144,7 @@ _sig("cmd_stage_synd", _I, _P, _I32, _SZ, _P, _SZ, _P)
 _sig("cmd_stage_synd_full", _I, _P, _I32, _SZ, _P, _SZ, _P)
 _sig("cmd_kem_dec", _I, _P, _SZ, _P, _P, _SZ, _P, _SZ)
 _sig("cmd_host_alloc", _I, _P, _SZ, _c.POINTER(_P))
+_sig("cmd_gen_encrypt_device", _I, _P, _SZ, _P, _P, _SZ, _I32, _P, _P)
 _sig("cmd_errorstream_format_positions", _I, _I32, _P, _SZ, _SZ, _P, _SZ, _c.POINTER(_SZ))
 _sig("cmd_host_free", _I, _P, _P)
 _sig("cmd_stage_bm", _I, _P, _I32, _SZ, _P, _P)
@@ -400,6 +401,18 @@ class Context:
                 for k in np.unique(keys)) if keys.size else 1
         rows = self.decrypt(keys, ct, mode=MODE_POSITIONS, ct_stride=ct_stride, e_stride=2 * t)
         return rows.view(np.uint16)
+
+    def gen_encrypt(self, key_ids: Sequence[int], seeds: Sequence[int], per_key: int, weight: int = -1):
+        """gen_encrypt(param, sk_k, seeds[k], per_key, weight) for every resident key k (one set),
+        computed on the GPU: (ct, e), key-major rows."""
+        keys = np.ascontiguousarray(key_ids, dtype=np.int32)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        p = params(self.key_param[int(keys[0])])
+        ct = np.zeros((keys.size * per_key, p.ct_bytes), np.uint8)
+        e = np.zeros((keys.size * per_key, p.e_bytes), np.uint8)
+        _check(_lib.cmd_gen_encrypt_device(self._h, keys.size, _ptr(keys), _ptr(sd), per_key, weight, _ptr(ct),
+                                           _ptr(e)), "cmd_gen_encrypt_device")
+        return ct, e
 
     def host_array(self, shape, dtype=np.uint8) -> np.ndarray:
         """A numpy array in pinned host memory (cmd_host_alloc): decrypt() moves such buffers by

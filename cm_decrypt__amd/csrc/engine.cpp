@@ -78,7 +78,7 @@ FusedSchedule default_schedule(int pidx)
 
 // Offsets of the arrays inside a key's single device allocation, and its size.
 struct KeyLayout {
-    size_t off_L, off_lb, off_bn, off_y, off_s, bytes;
+    size_t off_L, off_lb, off_bn, off_y, off_s, off_g, bytes;
 };
 
 static KeyLayout key_layout(int pidx)
@@ -92,7 +92,8 @@ static KeyLayout key_layout(int pidx)
     l.off_bn = l.off_lb + up((size_t)p.m * lbits_stride(pidx) * 4);
     l.off_y = l.off_bn + up((size_t)(2 * p.m - 1) * words * 4);
     l.off_s = l.off_y + up((size_t)p.m * words * 4);
-    l.bytes = l.off_s + up((size_t)p.n / 8);
+    l.off_g = l.off_s + up((size_t)p.n / 8);
+    l.bytes = l.off_g + up((size_t)2 * p.t);
     return l;
 }
 
@@ -738,6 +739,7 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
     slot.dev.benes = (const uint32_t *)(base + lay.off_bn);
     slot.dev.ypl = (const uint32_t *)(base + lay.off_y);
     slot.dev.s = (const uint8_t *)(base + lay.off_s);
+    slot.dev.gbytes = (const uint8_t *)(base + lay.off_g);
     slot.dev.pidx = pidx;
     slot.dev.lstride = lbits_stride(pidx);
     hipError_t e = hipMemcpyAsync(ctx->keystage.ptr, sk + 40, key_bytes, hipMemcpyHostToDevice, ctx->stream);
@@ -745,6 +747,8 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
     if (e == hipSuccess)
         e = hipMemcpyAsync(base + lay.off_s, (const char *)ctx->keystage.ptr + 2 * p.t + p.cond_bytes, (size_t)p.n / 8,
                            hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + lay.off_g, ctx->keystage.ptr, (size_t)2 * p.t, hipMemcpyDeviceToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         (void)hipFree(mem);

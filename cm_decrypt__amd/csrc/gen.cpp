@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/cm_decrypt.h"
+#include "engine.h"
 #include "host_gf.h"
 #include "keys.h"
 #include "params.h"
@@ -244,12 +245,40 @@ static bool encryptor_init(int pidx, const uint16_t *g, const uint16_t *L, Encry
     return true;
 }
 
+// Error positions of one ciphertext: the first `weight` entries of a partial Fisher-Yates
+// shuffle of 0 .. n-1 (entry i is final once step i is done); only the moved entries are
+// stored.
+static void draw_positions(Rng &rng, int n, int weight, uint16_t *out)
+{
+    std::vector<std::pair<int, int>> moved;  // index -> value, for indices that left identity
+    moved.reserve(2 * (size_t)weight);
+    auto get = [&](int i) {
+        for (const auto &m : moved)
+            if (m.first == i) return m.second;
+        return i;
+    };
+    auto put = [&](int i, int v) {
+        for (auto &m : moved)
+            if (m.first == i) {
+                m.second = v;
+                return;
+            }
+        moved.emplace_back(i, v);
+    };
+    for (int i = 0; i < weight; i++) {
+        const int j = i + (int)rng.below((uint32_t)(n - i));
+        const int vi = get(i), vj = get(j);
+        put(i, vj);
+        put(j, vi);
+        out[i] = (uint16_t)vj;
+    }
+}
+
 static void encrypt_one(const Encryptor &E, Rng &rng, int weight, uint8_t *ct, uint8_t *e)
 {
     const ParamDesc &p = kParams[E.pidx];
-    std::vector<int> pos(p.n);
-    for (int i = 0; i < p.n; i++) pos[i] = i;
-    for (int i = 0; i < weight; i++) std::swap(pos[i], pos[i + rng.below((uint32_t)(p.n - i))]);
+    std::vector<uint16_t> pos((size_t)weight);
+    draw_positions(rng, p.n, weight, pos.data());
     std::memset(e, 0, (size_t)p.n / 8);
     std::vector<uint64_t> y(E.words, 0);
     for (int i = 0; i < weight; i++) {
@@ -378,4 +407,103 @@ extern "C" int cmd_gen_keys(int32_t param_id, uint64_t base_seed, size_t n_keys,
     work();
     for (auto &th : pool) th.join();
     return status.load();
+}
+
+// ---- ciphertexts on the GPU --------------------------------------------------------------
+
+extern "C" int cmd_gen_encrypt_device(cmd_ctx *ctx, size_t n_keys, const int32_t *key_ids, const uint64_t *seeds,
+                                      size_t per_key, int32_t weight, uint8_t *ct, uint8_t *e)
+{
+    if (!ctx || (n_keys && (!key_ids || !seeds || (per_key && !ct)))) return CMD_ERR_ARG;
+    if (!n_keys || !per_key) return CMD_OK;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    int pidx = -1;
+    for (size_t k = 0; k < n_keys; k++) {
+        const int32_t id = key_ids[k];
+        if (id < 0 || (size_t)id >= ctx->keys.size() || !ctx->keys[id].used)
+            return set_error(CMD_ERR_KEY, "key %zu: id %d is not loaded", k, (int)id);
+        if (pidx < 0) pidx = ctx->keys[id].pidx;
+        if (ctx->keys[id].pidx != pidx) return set_error(CMD_ERR_ARG, "keys of one parameter set per call");
+    }
+    const ParamDesc &p = kParams[pidx];
+    if (weight < 0) weight = p.t;
+    if (weight > p.t) return CMD_ERR_ARG;
+    if (per_key > (size_t)INT32_MAX / 2) return CMD_ERR_ARG;
+    CMD_HIP(hipSetDevice(ctx->device));
+
+    // error positions: the host generator's random stream, drawn per key in parallel
+    std::vector<uint16_t> pos(n_keys * per_key * (size_t)weight);
+    {
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t k; (k = next.fetch_add(1)) < n_keys;) {
+                Rng rng(seeds[k] ^ 0xC1F3A5E7D2B49680ull ^ ((uint64_t)p.id << 20));
+                for (size_t i = 0; i < per_key; i++)
+                    draw_positions(rng, p.n, weight, &pos[(k * per_key + i) * (size_t)weight]);
+                if (e) {
+                    for (size_t i = 0; i < per_key; i++) {
+                        uint8_t *row = e + (k * per_key + i) * (size_t)(p.n / 8);
+                        std::memset(row, 0, (size_t)p.n / 8);
+                        const uint16_t *pp = &pos[(k * per_key + i) * (size_t)weight];
+                        for (int j = 0; j < weight; j++) row[pp[j] >> 3] |= (uint8_t)(1u << (pp[j] & 7));
+                    }
+                }
+            }
+        };
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned i = 1; i < nt && i < n_keys; i++) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+    }
+
+    const size_t mt = (size_t)p.m * p.t, cw = (mt + 31) / 32;
+    const size_t kBatch = 128;  // keys per device pass (bounds the per-key matrices)
+    int rc = CMD_OK;
+    for (size_t k0 = 0; k0 < n_keys && !rc; k0 += kBatch) {
+        const size_t nk = std::min(kBatch, n_keys - k0);
+        DevBuf d_cols, d_aug, d_ainvt, d_small, d_pos, d_ct;
+        const size_t ct_bytes = nk * per_key * (size_t)p.synd_bytes;
+        rc = dev_alloc(ctx, d_cols, nk * (size_t)p.n * cw * 4);
+        if (!rc) rc = dev_alloc(ctx, d_aug, nk * mt * 2 * cw * 4);
+        if (!rc) rc = dev_alloc(ctx, d_ainvt, nk * mt * cw * 4);
+        if (!rc) rc = dev_alloc(ctx, d_small, nk * 8);  // key ids, then singular flags
+        if (!rc) rc = dev_alloc(ctx, d_pos, nk * per_key * (size_t)weight * 2);
+        if (!rc) rc = dev_alloc(ctx, d_ct, ct_bytes);
+        std::vector<int32_t> flags(nk, 1);
+        if (!rc) {
+            GenArgs a;
+            a.pidx = pidx;
+            a.n_keys = (int32_t)nk;
+            a.keys = ctx->d_keys;
+            a.key_ids = (const int32_t *)d_small.ptr;
+            a.cols = (uint32_t *)d_cols.ptr;
+            a.aug = (uint32_t *)d_aug.ptr;
+            a.ainvt = (uint32_t *)d_ainvt.ptr;
+            a.singular = (int32_t *)d_small.ptr + nk;
+            a.per_key = (int32_t)per_key;
+            a.weight = weight;
+            a.pos = (const uint16_t *)d_pos.ptr;
+            a.ct = (uint8_t *)d_ct.ptr;
+            hipError_t he = hipMemcpyAsync(d_small.ptr, key_ids + k0, nk * 4, hipMemcpyHostToDevice, ctx->stream);
+            if (he == hipSuccess)
+                he = hipMemcpyAsync((int32_t *)d_small.ptr + nk, flags.data(), nk * 4, hipMemcpyHostToDevice, ctx->stream);
+            if (he == hipSuccess)
+                he = hipMemcpyAsync(d_pos.ptr, &pos[k0 * per_key * (size_t)weight], nk * per_key * (size_t)weight * 2,
+                                    hipMemcpyHostToDevice, ctx->stream);
+            if (he == hipSuccess) he = launch_gen_keys(a, ctx->stream);
+            if (he == hipSuccess) he = launch_gen_encrypt(a, ctx->stream);
+            if (he == hipSuccess)
+                he = hipMemcpyAsync(ct + k0 * per_key * (size_t)p.synd_bytes, d_ct.ptr, ct_bytes, hipMemcpyDeviceToHost,
+                                    ctx->stream);
+            if (he == hipSuccess)
+                he = hipMemcpyAsync(flags.data(), (int32_t *)d_small.ptr + nk, nk * 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+            if (he != hipSuccess) rc = hip_fail(he, "GPU ciphertext generator");
+        }
+        for (size_t k = 0; k < nk && !rc; k++)
+            if (flags[k]) rc = set_error(CMD_ERR_SINGULAR, "key %zu has no systematic form", k0 + k);
+        for (DevBuf *b : {&d_cols, &d_aug, &d_ainvt, &d_small, &d_pos, &d_ct}) dev_free(ctx, *b);
+    }
+    return rc;
 }

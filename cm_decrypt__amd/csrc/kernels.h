@@ -15,6 +15,7 @@ struct KeyDev {
     const uint32_t *benes;  // (2m-1) x 2^(m-5) swap masks of the key's Benes network, per 32-bit word
     const uint32_t *ypl;    // m x 2^(m-5) bit planes of 1/g(a)^2 in FFT order (word w, bit b: a = bitrev_m(32w+b))
     const uint8_t *s;       // n/8 bytes of the secret key's rejection value s
+    const uint8_t *gbytes;  // 2t bytes: the Goppa coefficients g_0 .. g_{t-1} as stored in the key
     int32_t pidx;
     int32_t lstride;        // words per bit-plane row
 };
@@ -133,6 +134,27 @@ struct PositionsArgs {
     uint64_t out_stride;
 };
 hipError_t launch_positions(const PositionsArgs &a, hipStream_t st);
+
+// ---- seeded ciphertext generator on the GPU (same bytes as the host generator) -----------
+// For n_keys resident keys of one set: H's columns (bit i*m + k of column j = bit k of
+// L_j^i / g(L_j)), [A | I] for A = the first mt columns, and A^-1 by Gauss-Jordan; then per
+// ciphertext ct = A^-1 (sum of the columns at its error positions).
+struct GenArgs {
+    int32_t pidx;
+    int32_t n_keys;
+    const KeyDev *keys;
+    const int32_t *key_ids;   // per key slot
+    uint32_t *cols;           // per key: n columns of cw words (cw = 32-bit words of mt bits)
+    uint32_t *aug;            // per key: mt rows of 2 cw words ([A | I], then [I | A^-1])
+    uint32_t *ainvt;          // per key: the mt columns of A^-1, cw words each
+    int32_t *singular;        // per key: 1 when A has no inverse
+    int32_t per_key;          // ciphertexts per key
+    int32_t weight;           // error positions per ciphertext
+    const uint16_t *pos;      // per ciphertext: `weight` positions (key-major order)
+    uint8_t *ct;              // per ciphertext: synd_bytes
+};
+hipError_t launch_gen_keys(const GenArgs &a, hipStream_t st);     // columns and inverses
+hipError_t launch_gen_encrypt(const GenArgs &a, hipStream_t st);  // ciphertexts
 
 // Host: fills the FFT constants for (m, l).
 void build_fft_tab(int m, int l, FftTab *out);

@@ -147,6 +147,15 @@ int cmd_key_export(cmd_ctx *ctx, int32_t key_id, uint16_t *table, uint16_t *L, u
 /* Device bytes one resident key of this set occupies. */
 int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes);
 
+/* The same ciphertexts as cmd_gen_encrypt, for resident keys of one set, computed on the GPU:
+ * ct rows k*per_key .. (k+1)*per_key-1 (SYND_BYTES each) are cmd_gen_encrypt(param, sk of
+ * key_ids[k], seeds[k], per_key, weight) and e (optional, n/8 bytes per row) its errors.
+ * Per key the parity-check columns and the inverse of their first mt columns (Gauss-Jordan,
+ * the work the reference's pk_gen.c:35-149 does on the CPU) are built on the device.
+ * CMD_ERR_SINGULAR when a key has no systematic form. */
+int cmd_gen_encrypt_device(cmd_ctx *ctx, size_t n_keys, const int32_t *key_ids, const uint64_t *seeds,
+                           size_t per_key, int32_t weight, uint8_t *ct, uint8_t *e);
+
 /* Batch decryption, host buffers (copied in and out, blocking).  Item i uses key key_ids[i]
  * and its ciphertext at ct + i*ct_stride; its error vector (n/8 bytes, the reference's
  * build_error_vector, decrypt.c:43-53) goes to e_out + i*e_stride, and the rest of each

@@ -423,3 +423,27 @@ def test_positions_mixed_batch_graph(ctx, keys):
                 assert np.array_equal(got[i, :p.t], w), f"replay {rep} row {i}"
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_gpu_generator_matches_host_generator(ctx, pid):
+    """cmd_gen_encrypt_device: per key the host generator's ciphertexts and errors, for
+    seeded keys, the shipped key (its golden ciphertexts), and weight below t."""
+    p = cm.params(pid)
+    sks = [cm.gen_keypair(pid, 300 + k) for k in range(3)] + [orc.shipped_sk(pid)]
+    kids = [ctx.load_key(pid, sk) for sk in sks]
+    try:
+        seeds = [11, 12, 13, 4242]
+        print(f"gpu generator pid={pid}", flush=True)
+        for weight, per in ((-1, 37), (p.t - 5, 6)):
+            ct, e = ctx.gen_encrypt(kids, seeds, per, weight)
+            for k, sk in enumerate(sks):
+                want_ct, want_e = cm.gen_encrypt(pid, sk, seeds[k], per, weight=weight)
+                assert np.array_equal(ct[k * per:(k + 1) * per], want_ct), f"key {k} weight {weight}"
+                assert np.array_equal(e[k * per:(k + 1) * per], want_e)
+        gold = np.load(os.path.join(orc.GOLDEN, f"golden_{pid}.npz"))
+        ct, _ = ctx.gen_encrypt(kids[3:], [4242], 5)
+        assert np.array_equal(ct, gold["ct"])
+    finally:
+        for k in kids:
+            ctx.unload_key(k)

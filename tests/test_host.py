@@ -213,3 +213,11 @@ def test_errorstream_from_positions_matches_bitmaps(pid):
             e[i, q >> 3] |= 1 << (q & 7)
     rows = _positions_rows(e, p.n, p.t)
     assert cm.errorstream_positions(pid, rows) == cm.errorstream(pid, e)
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_generator_reproduces_golden_fixtures(pid):
+    """The seeded generator still makes the committed golden ciphertexts and errors."""
+    g = np.load(os.path.join(orc.GOLDEN, f"golden_{pid}.npz"))
+    ct, e = cm.gen_encrypt(pid, orc.shipped_sk(pid), 4242, 5)
+    assert np.array_equal(ct, g["ct"]) and np.array_equal(e, g["e"])
