@@ -246,24 +246,26 @@ static bool encryptor_init(int pidx, const uint16_t *g, const uint16_t *L, Encry
 }
 
 // Error positions of one ciphertext: the first `weight` entries of a partial Fisher-Yates
-// shuffle of 0 .. n-1 (entry i is final once step i is done); only the moved entries are
-// stored.
-static void draw_positions(Rng &rng, int n, int weight, uint16_t *out)
+// shuffle of 0 .. n-1 (entry i is final once step i is done).  The shuffled array is virtual:
+// entries moved in this draw are marked with the draw's epoch, the rest equal their index.
+struct ShuffleScratch {
+    std::vector<int> val;
+    std::vector<uint32_t> epoch_of;
+    uint32_t epoch = 0;
+};
+
+static void draw_positions(Rng &rng, int n, int weight, uint16_t *out, ShuffleScratch &sc)
 {
-    std::vector<std::pair<int, int>> moved;  // index -> value, for indices that left identity
-    moved.reserve(2 * (size_t)weight);
-    auto get = [&](int i) {
-        for (const auto &m : moved)
-            if (m.first == i) return m.second;
-        return i;
-    };
+    if ((int)sc.val.size() < n) {
+        sc.val.assign(n, 0);
+        sc.epoch_of.assign(n, 0);
+        sc.epoch = 0;
+    }
+    const uint32_t ep = ++sc.epoch;
+    auto get = [&](int i) { return sc.epoch_of[i] == ep ? sc.val[i] : i; };
     auto put = [&](int i, int v) {
-        for (auto &m : moved)
-            if (m.first == i) {
-                m.second = v;
-                return;
-            }
-        moved.emplace_back(i, v);
+        sc.val[i] = v;
+        sc.epoch_of[i] = ep;
     };
     for (int i = 0; i < weight; i++) {
         const int j = i + (int)rng.below((uint32_t)(n - i));
@@ -274,11 +276,11 @@ static void draw_positions(Rng &rng, int n, int weight, uint16_t *out)
     }
 }
 
-static void encrypt_one(const Encryptor &E, Rng &rng, int weight, uint8_t *ct, uint8_t *e)
+static void encrypt_one(const Encryptor &E, Rng &rng, int weight, uint8_t *ct, uint8_t *e, ShuffleScratch &sc)
 {
     const ParamDesc &p = kParams[E.pidx];
     std::vector<uint16_t> pos((size_t)weight);
-    draw_positions(rng, p.n, weight, pos.data());
+    draw_positions(rng, p.n, weight, pos.data(), sc);
     std::memset(e, 0, (size_t)p.n / 8);
     std::vector<uint64_t> y(E.words, 0);
     for (int i = 0; i < weight; i++) {
@@ -364,8 +366,9 @@ extern "C" int cmd_gen_encrypt(int32_t param_id, const uint8_t *sk, size_t sk_le
     if (!encryptor_init(pidx, K.g.data(), K.L.data(), E)) return CMD_ERR_SINGULAR;
     Rng rng(seed ^ 0xC1F3A5E7D2B49680ull ^ ((uint64_t)p.id << 20));
     std::vector<uint8_t> tmp_e((size_t)p.n / 8);
+    ShuffleScratch sc;
     for (size_t i = 0; i < count; i++)
-        encrypt_one(E, rng, weight, ct + i * (size_t)p.synd_bytes, e ? e + i * (size_t)(p.n / 8) : tmp_e.data());
+        encrypt_one(E, rng, weight, ct + i * (size_t)p.synd_bytes, e ? e + i * (size_t)(p.n / 8) : tmp_e.data(), sc);
     return CMD_OK;
 }
 
@@ -436,10 +439,11 @@ extern "C" int cmd_gen_encrypt_device(cmd_ctx *ctx, size_t n_keys, const int32_t
     {
         std::atomic<size_t> next{0};
         auto work = [&]() {
+            ShuffleScratch sc;
             for (size_t k; (k = next.fetch_add(1)) < n_keys;) {
                 Rng rng(seeds[k] ^ 0xC1F3A5E7D2B49680ull ^ ((uint64_t)p.id << 20));
                 for (size_t i = 0; i < per_key; i++)
-                    draw_positions(rng, p.n, weight, &pos[(k * per_key + i) * (size_t)weight]);
+                    draw_positions(rng, p.n, weight, &pos[(k * per_key + i) * (size_t)weight], sc);
                 if (e) {
                     for (size_t i = 0; i < per_key; i++) {
                         uint8_t *row = e + (k * per_key + i) * (size_t)(p.n / 8);
