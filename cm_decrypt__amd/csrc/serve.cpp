@@ -58,9 +58,8 @@ constexpr uint32_t kBlockSize = 1u << kBlockBits;
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    uint8_t *h_ct = nullptr, *h_e = nullptr;  // pinned staging
-    char *h_plan = nullptr;                   // pinned plan staging
-    DevBuf dev;                               // d_ct | d_e | plan
+    uint8_t *h_in = nullptr, *h_e = nullptr;  // pinned staging: [ciphertexts | work list], results
+    DevBuf dev;                               // the same input image, then d_e
     std::vector<uint32_t> reqs;               // pool indices of the launch, in item order
     bool busy = false;                        // reserved by the worker (being launched or in flight)
     bool launched = false;                    // its completion event is recorded
@@ -87,7 +86,7 @@ struct cmd_server {
     std::vector<double> lat_us;
     double busy_since = -1, busy_us = 0;
     int busy_slots = 0;
-    size_t plan_bytes = 0;
+    size_t in_bytes = 0;  // capacity of a launch's input image
 
     Req &req(uint32_t idx) { return blocks[idx >> kBlockBits][idx & (kBlockSize - 1)]; }
 
@@ -152,42 +151,36 @@ int launch_slot(cmd_server *srv, Slot &s)
         plan_fused(ctx, per_set, count, key_ids.data(), tiles, items, item_key, &launch_tile);
         sched = launch_schedule(per_set, tiles, launch_tile);
     }
+    // one input image per launch, copied in one transfer: [ct rows | items | item keys |
+    // tiles | work-queue counter (zero)]
+    const size_t off_items = align256(count * kCtStride);
+    const size_t off_keys = off_items + align256(count * 4), off_tiles = off_keys + align256(count * 4);
+    const size_t off_ctr = off_tiles + align256(tiles.size() * sizeof(FusedTile));
+    const size_t in_bytes = off_ctr + 4;
     for (size_t i = 0; i < count; i++) {
         const Req &r = srv->req(s.reqs[i]);
-        std::memcpy(s.h_ct + i * kCtStride, r.ct, kParams[r.pidx].synd_bytes);
+        std::memcpy(s.h_in + i * kCtStride, r.ct, kParams[r.pidx].synd_bytes);
     }
-
-    const size_t mb = (size_t)srv->cfg.max_batch;
-    char *base = (char *)s.dev.ptr;
-    uint8_t *d_ct = (uint8_t *)base;
-    uint8_t *d_e = (uint8_t *)(base + align256(mb * kCtStride));
-    char *d_plan = base + align256(mb * kCtStride) + align256(mb * kEStride);
-    const size_t off_keys = align256(mb * 4), off_tiles = off_keys + align256(mb * 4);
-    const size_t off_ctr = off_tiles + align256(mb * sizeof(FusedTile));
-    std::memcpy(s.h_plan, items.data(), items.size() * 4);
-    std::memcpy(s.h_plan + off_keys, item_key.data(), item_key.size() * 4);
-    std::memcpy(s.h_plan + off_tiles, tiles.data(), tiles.size() * sizeof(FusedTile));
-
-    CMD_HIP(hipMemcpyAsync(d_ct, s.h_ct, count * kCtStride, hipMemcpyHostToDevice, s.stream));
-    CMD_HIP(hipMemcpyAsync(d_plan, s.h_plan, items.size() * 4, hipMemcpyHostToDevice, s.stream));
-    CMD_HIP(hipMemcpyAsync(d_plan + off_keys, s.h_plan + off_keys, item_key.size() * 4, hipMemcpyHostToDevice,
-                           s.stream));
-    CMD_HIP(hipMemcpyAsync(d_plan + off_tiles, s.h_plan + off_tiles, tiles.size() * sizeof(FusedTile),
-                           hipMemcpyHostToDevice, s.stream));
+    std::memcpy(s.h_in + off_items, items.data(), items.size() * 4);
+    std::memcpy(s.h_in + off_keys, item_key.data(), item_key.size() * 4);
+    std::memcpy(s.h_in + off_tiles, tiles.data(), tiles.size() * sizeof(FusedTile));
+    std::memset(s.h_in + off_ctr, 0, 4);
+    char *d_in = (char *)s.dev.ptr;
+    uint8_t *d_e = (uint8_t *)(d_in + srv->in_bytes);
+    CMD_HIP(hipMemcpyAsync(d_in, s.h_in, in_bytes, hipMemcpyHostToDevice, s.stream));
     FusedArgs a;
-    a.tiles = (const FusedTile *)(d_plan + off_tiles);
+    a.tiles = (const FusedTile *)(d_in + off_tiles);
     a.n_tiles = (int32_t)tiles.size();
     a.pad0 = 0;
-    a.items = (const int32_t *)d_plan;
-    a.item_key = (const int32_t *)(d_plan + off_keys);
+    a.items = (const int32_t *)(d_in + off_items);
+    a.item_key = (const int32_t *)(d_in + off_keys);
     a.keys = ctx->d_keys;
-    a.ct = d_ct;
+    a.ct = (const uint8_t *)d_in;
     a.e = d_e;
     a.ct_stride = kCtStride;
     a.e_stride = kEStride;
-    a.tile_counter = (uint32_t *)(d_plan + off_ctr);
+    a.tile_counter = (uint32_t *)(d_in + off_ctr);
     a.fft = ctx->d_fft;
-    CMD_HIP(launch_zero_u32(a.tile_counter, 1, s.stream));
     CMD_HIP(launch_fused(a, sched, ctx->n_cu, s.stream));
     CMD_HIP(hipMemcpyAsync(s.h_e, d_e, count * kEStride, hipMemcpyDeviceToHost, s.stream));
     CMD_HIP(hipEventRecord(s.done, s.stream));
@@ -320,8 +313,8 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     if (c.min_batch < 1) c.min_batch = 1;
     if (c.max_wait_us < 0) c.max_wait_us = 0;
     const size_t mb = (size_t)c.max_batch;
-    const size_t plan = align256(mb * 4) * 2 + align256(mb * sizeof(FusedTile)) + 256;
-    const size_t per_slot = align256(mb * kCtStride) + align256(mb * kEStride) + plan;
+    const size_t in_bytes = align256(mb * kCtStride) + align256(mb * 4) * 2 + align256(mb * sizeof(FusedTile)) + 256;
+    const size_t per_slot = in_bytes + align256(mb * kEStride);
     if (c.io_budget_bytes && per_slot * (size_t)c.max_inflight > c.io_budget_bytes)
         return set_error(CMD_ERR_BUDGET, "%d launch slots of %zu bytes exceed the I/O budget of %llu bytes",
                          c.max_inflight, per_slot, (unsigned long long)c.io_budget_bytes);
@@ -329,7 +322,7 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     std::unique_ptr<cmd_server> srv(new cmd_server());
     srv->ctx = ctx;
     srv->cfg = c;
-    srv->plan_bytes = plan;
+    srv->in_bytes = in_bytes;
     // pending requests plus those of every launch slot, rounded up to whole blocks
     const uint64_t cap = (uint64_t)c.queue_capacity + (uint64_t)c.max_inflight * (uint64_t)c.max_batch;
     const uint64_t nblocks = (cap + kBlockSize - 1) / kBlockSize;
@@ -344,9 +337,8 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     for (Slot &s : srv->slots) {
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_ct, mb * kCtStride, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_e, mb * kEStride, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_plan, plan, hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void **)&s.h_in, in_bytes, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&s.h_e, mb * kEStride, hipHostMallocDefault) != hipSuccess) {
             rc = set_error(CMD_ERR_HIP, "server slot allocation failed");
             break;
         }
@@ -357,9 +349,8 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
         for (Slot &s : srv->slots) {
             if (s.stream) (void)hipStreamDestroy(s.stream);
             if (s.done) (void)hipEventDestroy(s.done);
-            if (s.h_ct) (void)hipHostFree(s.h_ct);
+            if (s.h_in) (void)hipHostFree(s.h_in);
             if (s.h_e) (void)hipHostFree(s.h_e);
-            if (s.h_plan) (void)hipHostFree(s.h_plan);
             dev_free(ctx, s.dev);
         }
         return rc;
@@ -552,9 +543,8 @@ extern "C" int cmd_server_destroy(cmd_server *s)
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.done) (void)hipEventDestroy(sl.done);
-        if (sl.h_ct) (void)hipHostFree(sl.h_ct);
+        if (sl.h_in) (void)hipHostFree(sl.h_in);
         if (sl.h_e) (void)hipHostFree(sl.h_e);
-        if (sl.h_plan) (void)hipHostFree(sl.h_plan);
         dev_free(s->ctx, sl.dev);
     }
     delete s;  // every request was retired by the worker; the pool blocks go with the server
