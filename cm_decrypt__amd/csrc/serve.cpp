@@ -90,16 +90,33 @@ struct cmd_server {
 
     Req &req(uint32_t idx) { return blocks[idx >> kBlockBits][idx & (kBlockSize - 1)]; }
 
+    // A pool block with its pages already touched (allocated without mu held).
+    static std::unique_ptr<Req[]> new_block()
+    {
+        std::unique_ptr<Req[]> blk(new (std::nothrow) Req[kBlockSize]);
+        if (blk) {
+            volatile char *p = reinterpret_cast<char *>(blk.get());
+            for (size_t off = 0; off < sizeof(Req) * kBlockSize; off += 4096) p[off] = p[off];
+        }
+        return blk;
+    }
+
+    // Adds a block to the pool.  mu held.
+    void install(std::unique_ptr<Req[]> blk)
+    {
+        blocks[pool_size >> kBlockBits] = std::move(blk);
+        for (uint32_t i = kBlockSize; i-- > 0;) free_idx.push_back(pool_size + i);
+        pool_size += kBlockSize;
+    }
+
     // Pool index of a new request, or UINT32_MAX when the pool is exhausted.  mu held.
     uint32_t alloc()
     {
         if (free_idx.empty()) {
             if (pool_size >= pool_cap) return UINT32_MAX;
-            const uint32_t b = pool_size >> kBlockBits;
-            blocks[b].reset(new (std::nothrow) Req[kBlockSize]);
-            if (!blocks[b]) return UINT32_MAX;
-            for (uint32_t i = kBlockSize; i-- > 0;) free_idx.push_back(pool_size + i);
-            pool_size += kBlockSize;
+            std::unique_ptr<Req[]> blk(new (std::nothrow) Req[kBlockSize]);
+            if (!blk) return UINT32_MAX;
+            install(std::move(blk));
         }
         const uint32_t idx = free_idx.back();
         free_idx.pop_back();
@@ -331,6 +348,13 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     srv->pool_cap = (uint32_t)(nblocks << kBlockBits);
     srv->slots.resize(c.max_inflight);
     for (Slot &s : srv->slots) s.reqs.reserve(mb);
+    srv->lat_us.reserve(1 << 18);
+    // a first few blocks up front; submit grows the pool ahead of need
+    for (int b = 0; b < 4 && srv->pool_size < srv->pool_cap; b++) {
+        std::unique_ptr<Req[]> blk = cmd_server::new_block();
+        if (!blk) break;
+        srv->install(std::move(blk));
+    }
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     CMD_HIP(hipSetDevice(ctx->device));
     int rc = CMD_OK;
@@ -377,6 +401,18 @@ extern "C" int cmd_server_submit_batch(cmd_server *s, size_t count, const int32_
             if (ct_stride < (size_t)kParams[pidx[i]].synd_bytes)
                 return set_error(CMD_ERR_ARG, "ct_stride %zu below the ciphertext size of item %zu", ct_stride, i);
         }
+    }
+    // Grow the pool ahead of need without holding mu: a block is 4096 requests (5 MB), and
+    // allocating it under the lock would stall the worker's launches and retirements.
+    for (;;) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->free_idx.size() >= count + kBlockSize / 2 || s->pool_size >= s->pool_cap) break;
+        }
+        std::unique_ptr<Req[]> blk = cmd_server::new_block();
+        if (!blk) break;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->pool_size < s->pool_cap) s->install(std::move(blk));
     }
     size_t done = 0;
     int rc = CMD_OK;
