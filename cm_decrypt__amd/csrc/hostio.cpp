@@ -122,13 +122,24 @@ int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, con
         return set_error(CMD_ERR_ARG, "strides (%zu, %zu) below item sizes (%zu, %zu)", ct_stride, out_stride, max_ct,
                          need_out);
     if (ctx->io.empty()) {
-        ctx->io.resize(kIoSlots);
-        for (IoSlot &s : ctx->io) {
-            CMD_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
-            CMD_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-            CMD_HIP(hipEventCreate(&s.t0));
-            CMD_HIP(hipEventCreate(&s.t1));
+        // all slots or none: a failure leaves no half-made slot behind
+        std::vector<IoSlot> slots(kIoSlots);
+        hipError_t he = hipSuccess;
+        for (IoSlot &s : slots) {
+            if (he == hipSuccess) he = hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+            if (he == hipSuccess) he = hipEventCreate(&s.t0);
+            if (he == hipSuccess) he = hipEventCreate(&s.t1);
         }
+        if (he != hipSuccess) {
+            for (IoSlot &s : slots) {
+                for (hipEvent_t ev : {s.done, s.t0, s.t1})
+                    if (ev) (void)hipEventDestroy(ev);
+                if (s.st) (void)hipStreamDestroy(s.st);
+            }
+            return hip_fail(he, "pipeline streams");
+        }
+        ctx->io = std::move(slots);
     }
     // chunks of about kChunkItems, none below kMinChunkItems unless the call is smaller
     size_t n_chunks = std::max<size_t>(1, (count + kChunkItems - 1) / kChunkItems);
@@ -188,13 +199,18 @@ int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, con
             if (rc) break;
             dst = s.h_out;
         }
+        // errors end the loop (no early return): the chunks already in flight are drained below
         char *dp = (char *)s.d_plan.ptr;
-        CMD_HIP(hipMemcpyAsync(dp, s.h_plan, off_ctr, hipMemcpyHostToDevice, s.st));
-        CMD_HIP(hipMemcpyAsync(s.d_ct.ptr, src, n * ct_stride, hipMemcpyHostToDevice, s.st));
+        hipError_t he = hipMemcpyAsync(dp, s.h_plan, off_ctr, hipMemcpyHostToDevice, s.st);
+        if (he == hipSuccess) he = hipMemcpyAsync(s.d_ct.ptr, src, n * ct_stride, hipMemcpyHostToDevice, s.st);
         // rows come back whole: bytes past an item's output read as zero
-        CMD_HIP(hipMemsetAsync(s.d_out.ptr, 0, n * out_stride, s.st));
+        if (he == hipSuccess) he = hipMemsetAsync(s.d_out.ptr, 0, n * out_stride, s.st);
         s.timed = ctx->timing;
-        if (s.timed) CMD_HIP(hipEventRecord(s.t0, s.st));
+        if (he == hipSuccess && s.timed) he = hipEventRecord(s.t0, s.st);
+        if (he != hipSuccess) {
+            rc = hip_fail(he, "pipeline copy in");
+            break;
+        }
         rc = enqueue_fused(ctx, mode, ctx->sched, tiles, launch_tile, (const FusedTile *)(dp + off_tiles),
                            (const int32_t *)dp, (const int32_t *)(dp + off_keys), (uint32_t *)(dp + off_ctr),
                            (const uint8_t *)s.d_ct.ptr, ct_stride, (uint8_t *)s.d_out.ptr, out_stride,
@@ -202,9 +218,14 @@ int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, con
                                       (uint8_t *)s.d_msg.ptr},
                            s.st);
         if (rc) break;
-        if (s.timed) CMD_HIP(hipEventRecord(s.t1, s.st));
-        CMD_HIP(hipMemcpyAsync(dst, s.d_out.ptr, n * out_stride, hipMemcpyDeviceToHost, s.st));
-        CMD_HIP(hipEventRecord(s.done, s.st));
+        if (s.timed) he = hipEventRecord(s.t1, s.st);
+        if (he == hipSuccess) he = hipMemcpyAsync(dst, s.d_out.ptr, n * out_stride, hipMemcpyDeviceToHost, s.st);
+        if (he == hipSuccess) he = hipEventRecord(s.done, s.st);
+        if (he != hipSuccess) {
+            rc = hip_fail(he, "pipeline copy out");
+            (void)hipStreamSynchronize(s.st);
+            break;
+        }
         s.busy = true;
         s.first = first;
         s.count = n;
