@@ -151,6 +151,7 @@ _sig("cmd_stage_bm", _I, _P, _I32, _SZ, _P, _P)
 _sig("cmd_stage_root", _I, _P, _I32, _SZ, _P, _P, _SZ)
 _sig("cmd_schedule_get", _I, _P, _I32, _c.POINTER(Schedule))
 _sig("cmd_schedule_set", _I, _P, _I32, _c.POINTER(Schedule))
+_sig("cmd_set_tile_fit", _I, _P, _I32)
 _sig("cmd_autotune", _I, _P, _I32, _SZ, _P, _I32, _c.POINTER(Schedule), _c.POINTER(_c.c_double))
 _sig("cmd_schedule_save", _I, _P, _c.c_char_p)
 _sig("cmd_schedule_load", _I, _P, _c.c_char_p)
@@ -487,6 +488,10 @@ class Context:
         for k, v in kw.items():
             setattr(s, k, v)
         _check(_lib.cmd_schedule_set(self._h, param_id, _c.byref(s)), "cmd_schedule_set")
+
+    def set_tile_fit(self, on: bool):
+        """Narrow tiles of batches too small to fill the GPU (default on)."""
+        _check(_lib.cmd_set_tile_fit(self._h, int(on)), "cmd_set_tile_fit")
 
     def autotune(self, param_id: int, key_ids, reps: int = 3):
         keys = np.ascontiguousarray(key_ids, dtype=np.int32)

@@ -65,6 +65,8 @@ extern "C" int cmd_autotune(cmd_ctx *ctx, int32_t param_id, size_t count, const 
 
     const FusedSchedule saved = ctx->sched[pidx];
     FusedSchedule winner = saved;
+    const bool fit_saved = ctx->fit_tiles;
+    ctx->fit_tiles = false;  // time each candidate's own tile width
     double winner_us = 1e30;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
@@ -95,6 +97,7 @@ extern "C" int cmd_autotune(cmd_ctx *ctx, int32_t param_id, size_t count, const 
     (void)hipEventDestroy(e1);
     dev_free(ctx, dct);
     dev_free(ctx, de);
+    ctx->fit_tiles = fit_saved;
     ctx->sched[pidx] = rc ? saved : winner;
     if (rc) return rc;
     if (best) {

@@ -168,6 +168,30 @@ FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<Fus
     return s;
 }
 
+void fit_tiles(const cmd_ctx *ctx, const FusedSchedule in[], size_t count, const int32_t *key_ids,
+               FusedSchedule out[])
+{
+    size_t per_set[kNumParams] = {};
+    for (size_t i = 0; i < count; i++) {
+        const int32_t k = key_ids[i];
+        if (k >= 0 && (size_t)k < ctx->keys.size() && ctx->keys[k].used) per_set[ctx->keys[k].pidx]++;
+    }
+    for (int i = 0; i < kNumParams; i++) out[i] = in[i];
+    const size_t target = (size_t)ctx->n_cu * 4;
+    for (;;) {
+        size_t tiles = 0;
+        bool narrower = false;
+        for (int i = 0; i < kNumParams; i++) {
+            if (!per_set[i]) continue;
+            tiles += (per_set[i] + out[i].tile_cts - 1) / out[i].tile_cts;
+            narrower |= out[i].tile_cts > kMinFitTile;
+        }
+        if (tiles >= target || !narrower) break;
+        for (int i = 0; i < kNumParams; i++)
+            if (out[i].tile_cts > kMinFitTile) out[i].tile_cts >>= 1;
+    }
+}
+
 size_t out_row_bytes(int mode, size_t count, const int32_t *key_ids, const cmd_ctx *ctx)
 {
     size_t e = 0, t = 0;
@@ -825,7 +849,10 @@ static int make_batch(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *k
     b->ct_stride = ct_stride;
     b->e_stride = e_stride;
     b->use_graph = use_graph != 0;
-    for (int i = 0; i < kNumParams; i++) b->sched[i] = ctx->sched[i];
+    if (ctx->fit_tiles && mode != CMD_MODE_PLAIN && count)
+        fit_tiles(ctx, ctx->sched, count, key_ids, b->sched);
+    else
+        for (int i = 0; i < kNumParams; i++) b->sched[i] = ctx->sched[i];
     int rc = batch_plan(b.get(), key_ids);
     if (rc) {
         batch_release(b.get());
@@ -1107,6 +1134,14 @@ extern "C" int cmd_schedule_get(cmd_ctx *ctx, int32_t param_id, cmd_schedule *ou
     out->waves_per_ct = s.waves_per_ct;
     out->groups_per_cu = s.groups_per_cu;
     out->root_alg = s.root_alg;
+    return CMD_OK;
+}
+
+extern "C" int cmd_set_tile_fit(cmd_ctx *ctx, int32_t on)
+{
+    if (!ctx) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    ctx->fit_tiles = on != 0;
     return CMD_OK;
 }
 

@@ -84,6 +84,7 @@ struct cmd_ctx {
     std::vector<std::pair<uintptr_t, size_t>> host_pinned;  // cmd_host_alloc blocks
 
     bool timing = false;
+    bool fit_tiles = true;  // batches narrow their tiles when too small to fill the GPU (off while autotuning)
     double timing_us = 0.0;
     uint64_t timing_n = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -139,6 +140,13 @@ int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, co
 // Schedule a fused launch uses: tile width of the batch, waves/groups of its heaviest set.
 FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<FusedTile> &tiles, int launch_tile);
 
+// Schedules for a batch of `count` items: the tuned ones, with every tile width above
+// kMinFitTile halved until the batch has about four tiles per CU (or cannot narrow further).
+// A tile's Berlekamp-Massey costs the same for 4 or 32 ciphertexts, so a batch too small to
+// fill the GPU at the tuned width finishes sooner in more, narrower tiles.
+constexpr int kMinFitTile = 4;
+void fit_tiles(const cmd_ctx *ctx, const FusedSchedule in[], size_t count, const int32_t *key_ids,
+               FusedSchedule out[]);
 // Output row bytes a mode needs for these items (keys already validated).
 size_t out_row_bytes(int mode, size_t count, const int32_t *key_ids, const cmd_ctx *ctx);
 // Builds the plan of a batch (host arrays) and uploads it.

@@ -159,7 +159,9 @@ int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, con
         rc = io_finish(ctx, s, out, out_stride);
         if (rc) break;
         int launch_tile = 32;
-        rc = plan_fused(ctx, ctx->sched, n, key_ids + first, tiles, items, item_key, &launch_tile);
+        FusedSchedule sched[kNumParams];
+        fit_tiles(ctx, ctx->sched, n, key_ids + first, sched);
+        rc = plan_fused(ctx, sched, n, key_ids + first, tiles, items, item_key, &launch_tile);
         if (rc) break;
         // plan image: [items | item_key | tiles | counter]
         const size_t off_keys = (n * 4 + 255) / 256 * 256;
@@ -211,7 +213,7 @@ int io_decrypt(cmd_ctx *ctx, int mode, size_t count, const int32_t *key_ids, con
             rc = hip_fail(he, "pipeline copy in");
             break;
         }
-        rc = enqueue_fused(ctx, mode, ctx->sched, tiles, launch_tile, (const FusedTile *)(dp + off_tiles),
+        rc = enqueue_fused(ctx, mode, sched, tiles, launch_tile, (const FusedTile *)(dp + off_tiles),
                            (const int32_t *)dp, (const int32_t *)(dp + off_keys), (uint32_t *)(dp + off_ctr),
                            (const uint8_t *)s.d_ct.ptr, ct_stride, (uint8_t *)s.d_out.ptr, out_stride,
                            KemScratch{(uint8_t *)s.d_kem_e.ptr, kem_e_stride, (uint32_t *)s.d_ok.ptr,

@@ -234,6 +234,10 @@ typedef struct {
 
 int cmd_schedule_get(cmd_ctx *ctx, int32_t param_id, cmd_schedule *out);
 int cmd_schedule_set(cmd_ctx *ctx, int32_t param_id, const cmd_schedule *s);
+/* on != 0 (the default): a fused/KEM/positions batch too small to give about four tiles per
+ * CU at the schedules' tile widths halves them (not below 4) until it does.  Off: the
+ * schedules' widths apply as set. */
+int cmd_set_tile_fit(cmd_ctx *ctx, int32_t on);
 /* Times each candidate schedule on `count` synthetic items of the set (resident keys
  * required: key_ids of that set) and keeps the fastest; best_us receives its time. */
 int cmd_autotune(cmd_ctx *ctx, int32_t param_id, size_t count, const int32_t *key_ids,

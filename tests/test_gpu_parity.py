@@ -151,16 +151,22 @@ def test_fused_schedules_agree(ctx, keys, alg, tile, wpg, gpc):
             want.append(wx)
     saved = {pid: ctx.schedule(pid) for pid in PIDS}
     try:
+        ctx.set_tile_fit(False)  # the schedule's own tile width
         for pid in PIDS:
             ctx.set_schedule(pid, tile_cts=tile, waves_per_group=wpg, groups_per_cu=gpc, root_alg=alg)
         got = ctx.decrypt(np.array(kids, np.int32), np.stack(rows), mode=cm.MODE_FUSED, ct_stride=208, e_stride=1024)
+        ctx.set_tile_fit(True)
+        fitted = ctx.decrypt(np.array(kids, np.int32), np.stack(rows), mode=cm.MODE_FUSED, ct_stride=208,
+                             e_stride=1024)
     finally:
+        ctx.set_tile_fit(True)
         for pid, s in saved.items():
             ctx.set_schedule(pid, **s)
     want = np.stack(want)
     bad = [i for i in range(len(rows)) if not np.array_equal(got[i], want[i])]
     detail = [(i, int(np.nonzero(got[i] != want[i])[0][0])) for i in bad[:6]]
     assert not bad, f"{len(bad)} rows differ (row, first byte): {detail}"
+    assert np.array_equal(fitted, want), "narrowed tiles differ"
 
 
 @pytest.mark.parametrize("pid,K", [(348864, 63), (8192128, 63), (8192128, 127), (6688128, 127)])
@@ -177,10 +183,12 @@ def test_late_first_syndrome(ctx, keys, pid, K):
     assert np.array_equal(ctx.bm(pid, s), np.stack([orc.bm(pid, x) for x in s]))
     saved = ctx.schedule(pid)
     try:
+        ctx.set_tile_fit(False)
         for tile in (32, 1):
             ctx.set_schedule(pid, tile_cts=tile)
             for mode in (cm.MODE_PLAIN, cm.MODE_FUSED):
                 got = ctx.decrypt([kid] * len(ct), ct, mode=mode)
                 assert np.array_equal(got, want), (tile, mode)
     finally:
+        ctx.set_tile_fit(True)
         ctx.set_schedule(pid, **saved)
