@@ -160,10 +160,11 @@ int cmd_gen_encrypt_device(cmd_ctx *ctx, size_t n_keys, const int32_t *key_ids, 
  * and its ciphertext at ct + i*ct_stride; its error vector (n/8 bytes, the reference's
  * build_error_vector, decrypt.c:43-53) goes to e_out + i*e_stride, and the rest of each
  * e_stride row is zero.  Sets and keys may mix.  Replaces the reference's chunked batch
- * driver (GPU_Baseline/param/param8192128/src/cuda/Decrypt.cu:177-324): fused and KEM modes
- * run chunks of up to 65536 items through a three-stage pipeline (copy in, kernels, copy
- * out of different chunks overlap); buffers from cmd_host_alloc are copied by DMA
- * directly, others through pinned staging. */
+ * driver (GPU_Baseline/param/param8192128/src/cuda/Decrypt.cu:177-324): fused, KEM and
+ * positions modes run chunks of up to 65536 items through a three-stage pipeline (copy in,
+ * kernels, copy out of different chunks overlap); buffers from cmd_host_alloc are copied by
+ * DMA directly, others through pinned staging.  In CMD_MODE_KEM the row of an item is its
+ * 32-byte session key, in CMD_MODE_POSITIONS its t uint16 error positions. */
 int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
                 const uint8_t *ct, size_t ct_stride, uint8_t *e_out, size_t e_stride);
 
