@@ -446,8 +446,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "single-GPU build: bench.py runs on exactly one GPU; the per-GPU "
-                                   "workload and the sharding design are described in DESIGN.md",
+        print(json.dumps({"error": "single-GPU build: bench.py runs on exactly one GPU; decryptions are "
+                                   "independent per key, so G GPUs would each run this workload on 1/G "
+                                   "of the keys",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
