@@ -30,6 +30,21 @@ __device__ __forceinline__ uint32_t bitmask(uint32_t x, int k)
     return (uint32_t)__builtin_amdgcn_sbfe((int)x, k, 1);
 }
 
+// gfx950's three-input v_bitop3_b32 evaluates any function of three operands in one
+// instruction; its 8-bit table is indexed by (src0 << 2) | (src1 << 1) | src2.
+// 0x6A: src2 ^ (src0 & src1)      0x96: src0 ^ src1 ^ src2
+// Written as builtins so that every AND-XOR term of a product is one instruction (left to
+// itself the compiler re-associates the sums into trees of separate ANDs and XORs).
+__device__ __forceinline__ uint32_t and_xor(uint32_t acc, uint32_t x, uint32_t y)
+{
+    return __builtin_amdgcn_bitop3_b32(x, y, acc, 0x6A);
+}
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Reduces a product of degree <= 2M-2 modulo the field polynomial.
 template <int M>
 __device__ __forceinline__ uint32_t gf_reduce(uint32_t v)
@@ -49,14 +64,16 @@ __device__ __forceinline__ uint32_t gf_reduce(uint32_t v)
     return v;
 }
 
-// Scalar product (shift-and-add over the bits of b).
+// Scalar product (shift-and-add over the bits of b, one AND-XOR per bit, two partial sums).
 template <int M>
 __device__ __forceinline__ uint32_t gf_mul(uint32_t a, uint32_t b)
 {
-    uint32_t acc = 0;
+    uint32_t acc0 = 0, acc1 = 0;
 #pragma unroll
-    for (int i = 0; i < M; i++) acc ^= (a << i) & bitmask(b, i);
-    return gf_reduce<M>(acc);
+    for (int i = 0; i < M; i += 2) acc0 = and_xor(acc0, a << i, bitmask(b, i));
+#pragma unroll
+    for (int i = 1; i < M; i += 2) acc1 = and_xor(acc1, a << i, bitmask(b, i));
+    return gf_reduce<M>(acc0 ^ acc1);
 }
 
 // x * a for a reduced element
@@ -80,10 +97,12 @@ __device__ __forceinline__ void gf_mul_cols(uint32_t a, uint32_t col[M])
 template <int M>
 __device__ __forceinline__ uint32_t gf_mul_by_cols(const uint32_t col[M], uint32_t b)
 {
-    uint32_t acc = 0;
+    uint32_t acc0 = 0, acc1 = 0;
 #pragma unroll
-    for (int k = 0; k < M; k++) acc ^= col[k] & bitmask(b, k);
-    return acc;
+    for (int k = 0; k < M; k += 2) acc0 = and_xor(acc0, col[k], bitmask(b, k));
+#pragma unroll
+    for (int k = 1; k < M; k += 2) acc1 = and_xor(acc1, col[k], bitmask(b, k));
+    return acc0 ^ acc1;
 }
 
 // a^(2^M - 2): the inverse, with inv(0) == 0.
@@ -115,21 +134,6 @@ __device__ __forceinline__ uint32_t gf_inv(uint32_t a)
 }
 
 // ---- bit-sliced ------------------------------------------------------------------------
-// gfx950's three-input v_bitop3_b32 evaluates any function of three operands in one
-// instruction; its 8-bit table is indexed by (src0 << 2) | (src1 << 1) | src2.
-// 0x6A: src2 ^ (src0 & src1)      0x96: src0 ^ src1 ^ src2
-// Written as builtins so that every AND-XOR term of a product is one instruction (left to
-// itself the compiler re-associates the sums into trees of separate ANDs and XORs).
-__device__ __forceinline__ uint32_t and_xor(uint32_t acc, uint32_t x, uint32_t y)
-{
-    return __builtin_amdgcn_bitop3_b32(x, y, acc, 0x6A);
-}
-
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
-{
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-
 // Which high planes of an unreduced product land in plane k after reduction: x^s mod the
 // field polynomial for s = M .. 2M-2, as a table computed at compile time.
 template <int M>
