@@ -64,16 +64,25 @@ __device__ __forceinline__ uint32_t gf_reduce(uint32_t v)
     return v;
 }
 
-// Scalar product (shift-and-add over the bits of b, one AND-XOR per bit, two partial sums).
+// Carry-less product of two reduced elements (degree <= 2M-2, not reduced): shift-and-add
+// over the bits of b, one AND-XOR per bit, two partial sums.  Sums of such products reduce
+// once (the reduction is linear).
 template <int M>
-__device__ __forceinline__ uint32_t gf_mul(uint32_t a, uint32_t b)
+__device__ __forceinline__ uint32_t gf_clmul(uint32_t a, uint32_t b)
 {
     uint32_t acc0 = 0, acc1 = 0;
 #pragma unroll
     for (int i = 0; i < M; i += 2) acc0 = and_xor(acc0, a << i, bitmask(b, i));
 #pragma unroll
     for (int i = 1; i < M; i += 2) acc1 = and_xor(acc1, a << i, bitmask(b, i));
-    return gf_reduce<M>(acc0 ^ acc1);
+    return acc0 ^ acc1;
+}
+
+// Scalar product.
+template <int M>
+__device__ __forceinline__ uint32_t gf_mul(uint32_t a, uint32_t b)
+{
+    return gf_reduce<M>(gf_clmul<M>(a, b));
 }
 
 // x * a for a reduced element

@@ -807,20 +807,22 @@ struct FusedTileKernel {
                 bv[b] = (b == 0 && lane == 1) ? 1u : 0u;  // B = x
             }
             uint32_t bb = 1, L = 0;
+            uint32_t colb[M];  // multiplication table of bb, renewed when bb changes
+            gf_mul_cols<M>(bb, colb);
 #pragma unroll 1
             for (int n = 0; n < 2 * T; n++) {
+                // the lanes' products are summed unreduced; the sum reduces once
                 uint32_t part = 0;
 #pragma unroll
                 for (int b = 0; b < NBL; b++) {
                     if (64 * b > n + 1) continue;
                     const int j = n - (lane + 64 * b);
                     const uint32_t sv = j >= 0 ? (uint32_t)Sx[j] : 0u;
-                    part ^= gf_mul<M>(c[b], sv);
+                    part ^= gf_clmul<M>(c[b], sv);
                 }
-                const uint32_t d = uni(wave_xor_all(part));
+                const uint32_t d = gf_reduce<M>(uni(wave_xor_all(part)));
                 const bool mle = d != 0 && 2 * L <= (uint32_t)n;
-                uint32_t colb[M], cold[M];
-                gf_mul_cols<M>(bb, colb);
+                uint32_t cold[M];
                 gf_mul_cols<M>(d, cold);
 #pragma unroll
                 for (int b = 0; b < NBL; b++) {
@@ -840,6 +842,8 @@ struct FusedTileKernel {
                 if (mle) {
                     L = (uint32_t)n + 1 - L;
                     bb = d;
+#pragma unroll
+                    for (int k = 0; k < M; k++) colb[k] = cold[k];
                 }
             }
 #pragma unroll
