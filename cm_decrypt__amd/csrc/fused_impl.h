@@ -563,17 +563,27 @@ struct FusedTileKernel {
     }
 
     // ---- stage 1: syndromes into LDS bit planes ---------------------------------------------
-    static __device__ __forceinline__ void syndrome(const FusedArgs &a, const FusedTile &tl, bool one_key, uint32_t *lds,
-                                    int lane)
+    // A tile's ciphertexts fall into runs of one key (the planner orders items by key).  Each
+    // run costs one streamed pass over its key's whole table, shared by the run's ciphertexts;
+    // a tile of many short runs (mixed serving batches) instead reads, ciphertext by
+    // ciphertext, only the rows of set bits.
+    static __device__ __forceinline__ void syndrome(const FusedArgs &a, const FusedTile &tl, uint32_t *lds, int lane)
     {
         // row 0: the zero row read for s_{negative}
         if (lane < kRowWords) lds[lane] = 0;
         const int chunks = (tl.count + CH - 1) / CH;
         for (int c = 0; c < chunks; c++) {
             const int x0 = c * CH;
-            const uint8_t *my_ct = nullptr;
-            if (lane < CH && x0 + lane < tl.count)
-                my_ct = a.ct + (size_t)a.items[tl.first + x0 + lane] * a.ct_stride;
+            const bool live = lane < CH && x0 + lane < tl.count;
+            const int my_key = live ? a.item_key[tl.first + x0 + lane] : -1;
+            const uint8_t *ct_ptr = live ? a.ct + (size_t)a.items[tl.first + x0 + lane] * a.ct_stride : nullptr;
+            const int n_live = tl.count - x0 < CH ? tl.count - x0 : CH;
+            // distinct keys of the chunk (wave-uniform)
+            int runs = 0;
+            for (uint64_t rem = __ballot(live); rem; runs++) {
+                const int k = __builtin_amdgcn_readlane(my_key, __builtin_ctzll(rem));
+                rem &= ~__ballot(my_key == k);
+            }
 
             uint32_t acc[CH][R];
 #pragma unroll
@@ -581,29 +591,36 @@ struct FusedTileKernel {
 #pragma unroll
                 for (int r = 0; r < R; r++) acc[x][r] = 0;
 
-            if (one_key) {
-                // Row groups are software-pipelined through two register buffers: the loads of
-                // the next group are in flight while the current one is added, hiding the L2
-                // latency.  One iteration is one ciphertext word (four groups); 6960119's last
-                // word has two.
-                const KeyDev *kd = &a.keys[a.item_key[tl.first + x0]];
-                gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
-                uint32_t va[8][R], vb[8][R], cw[CH];
-                load_group(rows, 0, lane, va);
-                for (int w = 0; w < WORDS; w++) {
-                    const uint32_t mine = ct_word(my_ct, w);
+            if (tl.count > SCALAR_BM && 2 * runs <= n_live) {
+                for (uint64_t rem = __ballot(live); rem;) {
+                    const int key = __builtin_amdgcn_readlane(my_key, __builtin_ctzll(rem));
+                    const bool in_run = my_key == key;
+                    rem &= ~__ballot(in_run);
+                    // ciphertexts outside the run read as zero words and add nothing
+                    const uint8_t *my_ct = in_run ? ct_ptr : nullptr;
+                    // Row groups are software-pipelined through two register buffers: the loads of
+                    // the next group are in flight while the current one is added, hiding the L2
+                    // latency.  One iteration is one ciphertext word (four groups); 6960119's last
+                    // word has two.
+                    const KeyDev *kd = &a.keys[key];
+                    gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
+                    uint32_t va[8][R], vb[8][R], cw[CH];
+                    load_group(rows, 0, lane, va);
+                    for (int w = 0; w < WORDS; w++) {
+                        const uint32_t mine = ct_word(my_ct, w);
 #pragma unroll
-                    for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
-                    const int g = 4 * w;
-                    if (g + 1 < NGROUP) load_group(rows, g + 1, lane, vb);
-                    add_group<0>(va, cw, acc);
-                    if (g + 2 < NGROUP) load_group(rows, g + 2, lane, va);
-                    if (g + 1 < NGROUP) add_group<8>(vb, cw, acc);
-                    if (g + 2 >= NGROUP) break;
-                    load_group(rows, g + 3, lane, vb);
-                    add_group<16>(va, cw, acc);
-                    if (g + 4 < NGROUP) load_group(rows, g + 4, lane, va);
-                    add_group<24>(vb, cw, acc);
+                        for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
+                        const int g = 4 * w;
+                        if (g + 1 < NGROUP) load_group(rows, g + 1, lane, vb);
+                        add_group<0>(va, cw, acc);
+                        if (g + 2 < NGROUP) load_group(rows, g + 2, lane, va);
+                        if (g + 1 < NGROUP) add_group<8>(vb, cw, acc);
+                        if (g + 2 >= NGROUP) break;
+                        load_group(rows, g + 3, lane, vb);
+                        add_group<16>(va, cw, acc);
+                        if (g + 4 < NGROUP) load_group(rows, g + 4, lane, va);
+                        add_group<24>(vb, cw, acc);
+                    }
                 }
             } else {
                 // one key per ciphertext: each row feeds one accumulator.  The ciphertext is
@@ -1032,14 +1049,7 @@ struct FusedTileKernel {
 
     static __device__ __forceinline__ void run(const FusedArgs &a, const FusedTile &tl, uint32_t *lds, int lane, bool e_aligned)
     {
-        bool one_key = true;
-        {
-            const int k0 = a.item_key[tl.first];
-            for (int x = lane; x < tl.count; x += 64) one_key &= (a.item_key[tl.first + x] == k0);
-            one_key = __all(one_key);
-        }
-        // tiles of a few ciphertexts read only the rows of set bits, ciphertext by ciphertext
-        syndrome(a, tl, one_key && tl.count > SCALAR_BM, lds, lane);
+        syndrome(a, tl, lds, lane);
         wave_sync();
         uint32_t C[NBL][M];
         if (tl.count <= SCALAR_BM)

@@ -117,6 +117,40 @@ def test_decrypt_mixed_sets_and_keys(ctx, keys, shipped, mode):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("pid", [348864, 6960119])
+def test_key_runs_inside_tiles(ctx, pid):
+    """Tiles holding ciphertexts of several keys (items are planned in key order): runs of one
+    key share a table pass, tiles of many short runs read ciphertext by ciphertext, a tile of at
+    most three ciphertexts takes the scalar path; every row against the oracle, on distinct keys
+    so that a table taken from the wrong key shows."""
+    sks = cm.gen_keys(pid, 91, 9, threads=8)
+    kids = [ctx.load_key(pid, s) for s in sks]
+    # 32-wide tiles: [1,2,3,5,8,13] (six runs), [1,30,1] (three), [3] (scalar); then one tile
+    # of nine single-ciphertext runs; then [16,16] (two runs of half a tile)
+    layouts = [[1, 2, 3, 5, 8, 13, 1, 30, 4], [1] * 9, [16, 16]]
+    saved = ctx.schedule(pid)
+    try:
+        ctx.set_tile_fit(False)
+        ctx.set_schedule(pid, tile_cts=32, waves_per_group=1, groups_per_cu=8, root_alg=1)
+        for li, counts in enumerate(layouts):
+            ct_rows, kk, want = [], [], []
+            for k, n in enumerate(counts):
+                ct, _ = cm.gen_encrypt(pid, sks[k], 300 + 10 * li + k, n)
+                want.append(orc.decrypt(pid, sks[k], ct))
+                ct_rows.append(ct)
+                kk += [kids[k]] * len(ct)
+            ct = np.concatenate(ct_rows)
+            want = np.concatenate(want)
+            order = np.random.default_rng(li).permutation(len(ct))
+            got = ctx.decrypt(np.array(kk, np.int32)[order], ct[order], mode=cm.MODE_FUSED)
+            assert np.array_equal(got, want[order]), f"layout {counts}"
+    finally:
+        ctx.set_tile_fit(True)
+        ctx.set_schedule(pid, **saved)
+        for k in kids:
+            ctx.unload_key(k)
+
+
 @pytest.mark.parametrize("pid", PIDS)
 def test_weight_below_t(ctx, keys, pid):
     """w < t leaves a factor x^(t-w) in the reversed locator: a support element equal to 0
