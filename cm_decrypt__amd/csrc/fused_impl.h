@@ -681,15 +681,33 @@ struct FusedTileKernel {
     }
 
     // ---- stage 2: bit-sliced Berlekamp-Massey; leaves coefficient rows in LDS ----------------
-    // Lane l holds coefficient i = l + 64b of C and B (b < NBL).  Before iteration n's update C
+    // Lane l holds coefficient i = l + 64b of C and B (b < NBV).  Before iteration n's update C
     // has degree <= n and B degree <= n+1 (B = x^(n+1) until the first nonzero discrepancy), so
     // block b joins the products once 64b <= n+1 and the shift of B once 64b <= n+2 (public
     // bounds).
+    //
+    // When t is a multiple of 64 the top coefficient t would occupy a lane block of its own,
+    // one live lane doing a whole block's products.  It is held transposed instead: lane k
+    // (k < m) holds plane k of C_t (ct) and of B_t (bt), and its products with the wave-uniform
+    // field elements b, d and s_{n-t} are Horner evaluations over those planes, one shift up the
+    // lanes (multiplication by x, the overflowing plane folded back by x^m = low(x)) and one
+    // AND-XOR per plane of the scalar.  Lanes >= m hold values that only move upwards and are
+    // never read.
+    static constexpr bool TOPX = (T % 64) == 0;
+    static constexpr int NBV = TOPX ? T / 64 : NBL;  // lane blocks of coefficients
+
+    // x * v for a transposed element v (planes in lanes 0..M-1); pm has the planes of low(x).
+    static __device__ __forceinline__ uint32_t tx_mulx(uint32_t v, uint32_t pm)
+    {
+        const uint32_t top = (uint32_t)__builtin_amdgcn_readlane((int)v, M - 1);
+        return and_xor(shift_up1_zero(v), pm, top);
+    }
+
     static __device__ __forceinline__ void berlekamp_massey(uint32_t *lds, int lane, uint32_t (&C)[NBL][M])
     {
-        uint32_t B[NBL][M];
+        uint32_t B[NBV][M];
 #pragma unroll
-        for (int b = 0; b < NBL; b++)
+        for (int b = 0; b < NBV; b++)
 #pragma unroll
             for (int k = 0; k < M; k++) {
                 C[b][k] = (b == 0 && k == 0 && lane == 0) ? ~0u : 0u;  // C = 1
@@ -702,7 +720,9 @@ struct FusedTileKernel {
         for (int q = 0; q < 9; q++) Lp[q] = 0;
 
         // coefficients above T (last block) are dropped like the reference's fixed t+1 array
-        const uint32_t keep_last = (lane + 64 * (NBL - 1) <= T) ? ~0u : 0u;
+        [[maybe_unused]] const uint32_t keep_last = (lane + 64 * (NBV - 1) <= T) ? ~0u : 0u;
+        [[maybe_unused]] const uint32_t pm = (lane < M && ((Modulus<M>::low >> lane) & 1u)) ? ~0u : 0u;
+        [[maybe_unused]] uint32_t ct = 0, bt = 0;  // TOPX: planes of C_t and B_t, transposed
         const uint4 *rows4 = reinterpret_cast<const uint4 *>(lds);
 
 #pragma unroll 1
@@ -712,7 +732,7 @@ struct FusedTileKernel {
 #pragma unroll
             for (int s2 = 0; s2 < 2 * M - 1; s2++) p[s2] = 0;
 #pragma unroll
-            for (int b = 0; b < NBL; b++) {
+            for (int b = 0; b < NBV; b++) {
                 if (64 * b > n + 1) continue;
                 const int i = lane + 64 * b;
                 const int row = (n - i + 1) > 0 ? (n - i + 1) : 0;
@@ -728,14 +748,40 @@ struct FusedTileKernel {
             bs_fold<M>(p, part);
 #pragma unroll
             for (int k = 0; k < M; k++) d[k] = wave_xor_all(part[k]);
+            if constexpr (TOPX) {
+                if (n >= T) {
+                    // + C_t s_{n-t}: s_{n-t} (row n-t+1) as wave-uniform planes, Horner over them
+                    const int row = n - T + 1;
+                    uint32_t sv[16];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint4 w = rows4[row * 4 + q];
+                        sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
+                    }
+                    uint32_t acc = and_xor(0u, ct, uni(sv[M - 1]));
+#pragma unroll
+                    for (int a = M - 2; a >= 0; a--) acc = and_xor(tx_mulx(acc, pm), ct, uni(sv[a]));
+#pragma unroll
+                    for (int k = 0; k < M; k++) d[k] ^= (uint32_t)__builtin_amdgcn_readlane((int)acc, k);
+                }
+            }
             uint32_t mne = 0;
 #pragma unroll
             for (int k = 0; k < M; k++) mne |= d[k];
             const uint32_t mle = mne & bs_le_const(Lp, (uint32_t)n >> 1);
 
             // C' = b*C + d*B ; B' = mle ? C : B ; then B <- x*B
+            if constexpr (TOPX) {
+                if (n + 1 >= T) {
+                    // C_t' = b C_t + d B_t (B_t' comes from the shift below)
+                    uint32_t acc = and_xor(and_xor(0u, ct, bb[M - 1]), bt, d[M - 1]);
 #pragma unroll
-            for (int b = NBL - 1; b >= 0; b--) {
+                    for (int a = M - 2; a >= 0; a--) acc = and_xor(and_xor(tx_mulx(acc, pm), ct, bb[a]), bt, d[a]);
+                    ct = acc;
+                }
+            }
+#pragma unroll
+            for (int b = NBV - 1; b >= 0; b--) {
                 if (64 * b > n + 1) continue;
                 uint32_t cn[M];
                 bs_mul_add2<M>(bb, C[b], d, B[b], cn);
@@ -745,10 +791,21 @@ struct FusedTileKernel {
                     C[b][k] = cn[k];
                 }
             }
+            // the shift moves coefficient t-1 of the selected B into B_t (TOPX); the old B_t
+            // moves past t and is dropped
+            if constexpr (TOPX) {
+                if (n + 2 >= T) {
+#pragma unroll
+                    for (int k = 0; k < M; k++) {
+                        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)B[NBV - 1][k], 63);
+                        bt = lane == k ? c : bt;
+                    }
+                }
+            }
             // shift the highest block first so each carry is read before its source moves; B has
             // degree <= n+1 here, so block b can receive a carry once 64b <= n+2
 #pragma unroll
-            for (int b = NBL - 1; b >= 0; b--) {
+            for (int b = NBV - 1; b >= 0; b--) {
                 if (64 * b > n + 2) continue;
 #pragma unroll
                 for (int k = 0; k < M; k++) {
@@ -760,8 +817,10 @@ struct FusedTileKernel {
                     }
                 }
             }
+            if constexpr (!TOPX) {
 #pragma unroll
-            for (int k = 0; k < M; k++) B[NBL - 1][k] &= keep_last;
+                for (int k = 0; k < M; k++) B[NBV - 1][k] &= keep_last;
+            }
             // L, b
             uint32_t Ln[9];
             bs_const_sub((uint32_t)n + 1, Lp, Ln);
@@ -769,6 +828,14 @@ struct FusedTileKernel {
             for (int q = 0; q < 9; q++) Lp[q] = (Ln[q] & mle) | (Lp[q] & ~mle);
 #pragma unroll
             for (int k = 0; k < M; k++) bb[k] = (d[k] & mle) | (bb[k] & ~mle);
+        }
+        if constexpr (TOPX) {
+            // C_t into lane 0 of the last block
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)ct, k);
+                C[NBL - 1][k] = lane == 0 ? w : 0u;
+            }
         }
     }
 
