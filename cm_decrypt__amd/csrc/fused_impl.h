@@ -527,28 +527,58 @@ struct FusedTileKernel {
             for (int r = 0; r < R; r++) v[q][r] = (lane + 64 * r < PAIRS) ? base[(size_t)q * PAIRS + lane + 64 * r] : 0u;
     }
 
-    // The eight rows of group g into the accumulators of the CH ciphertexts of a single-key
-    // chunk: row 8g+q is added where bit BIT0+q of the ciphertext's word is set (BIT0 = 8 (g mod
-    // 4), a constant so the masks stay scalar).  The masks are wave-uniform and made sixteen at
-    // a time in SGPRs (s_bfe), so each term costs one v_bitop3.
+    // The eight rows of group g into the accumulators of the CH ciphertexts of a run of one
+    // key: row 8g+q is added where bit BIT0+q of the ciphertext's word is set (BIT0 = 8 (g mod
+    // 4)).  Four rows at a time ("four Russians"): the 16 XOR combinations of rows 4h .. 4h+3
+    // are written to LDS (table h, entry c, lane-major; Gray-code order, one XOR each), then
+    // every ciphertext adds the entry its nibble selects: one LDS read per nibble and one
+    // three-input XOR per word in place of four wave-uniform masks and 4R AND-XORs.  Entry 0
+    // of each table is zero, written once per tile.  Ciphertext bits are public, so the
+    // data-dependent LDS address leaks nothing.
+    static __device__ __forceinline__ void tab_put(uint32_t *tab, int e, int lane, const uint32_t (&v)[R])
+    {
+        if constexpr (R == 2)
+            reinterpret_cast<uint2 *>(tab)[e * 64 + lane] = make_uint2(v[0], v[1]);
+        else
+            tab[e * 64 + lane] = v[0];
+    }
+    static __device__ __forceinline__ uint2 tab_get(const uint32_t *tab, uint32_t e, int lane)
+    {
+        if constexpr (R == 2)
+            return reinterpret_cast<const uint2 *>(tab)[e * 64 + lane];
+        else
+            return make_uint2(tab[e * 64 + lane], 0u);
+    }
+
     template <int BIT0>
     static __device__ __forceinline__ void add_group(const uint32_t (&v)[8][R], const uint32_t (&cw)[CH],
-                                                     uint32_t (&acc)[CH][R])
+                                                     uint32_t (&acc)[CH][R], uint32_t *tab, int lane)
     {
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
+        for (int h = 0; h < 2; h++) {
+            uint32_t cur[R];
 #pragma unroll
-            for (int x0 = 0; x0 < CH; x0 += 16) {
-                uint32_t msk[16];
+            for (int r = 0; r < R; r++) cur[r] = 0;
 #pragma unroll
-                for (int x = 0; x < 16; x++) msk[x] = (uint32_t)__builtin_amdgcn_sbfe((int)cw[x0 + x], BIT0 + q, 1);
+            for (int i = 1; i < 16; i++) {
+                const int q = 4 * h + __builtin_ctz(i);
 #pragma unroll
-                for (int x = 0; x < 16; x++)
-#pragma unroll
-                    for (int r = 0; r < R; r++) acc[x0 + x][r] = and_xor(acc[x0 + x][r], v[q][r], msk[x]);
+                for (int r = 0; r < R; r++) cur[r] ^= v[q][r];
+                tab_put(tab, 16 * h + (i ^ (i >> 1)), lane, cur);
             }
         }
+#pragma unroll
+        for (int x = 0; x < CH; x++) {
+            const uint2 lo = tab_get(tab, (cw[x] >> BIT0) & 15u, lane);
+            const uint2 hi = tab_get(tab, 16u + ((cw[x] >> (BIT0 + 4)) & 15u), lane);
+            acc[x][0] = xor3(acc[x][0], lo.x, hi.x);
+            if constexpr (R == 2) acc[x][R - 1] = xor3(acc[x][R - 1], lo.y, hi.y);
+            // pin the sum here: left free, the compiler defers these XORs and keeps the
+            // loaded entries of many groups live, which overflows the register file
+            asm volatile("" : "+v"(acc[x][0]), "+v"(acc[x][R - 1]));
+        }
     }
+    static_assert(32 * 64 * R <= kWaveLds, "four-row tables must fit the wave's LDS");
 
     // Bytes 4w .. 4w+3 of a ciphertext (those that exist) as a little-endian word.
     static __device__ __forceinline__ uint32_t ct_word(const uint8_t *c, int w)
@@ -569,8 +599,6 @@ struct FusedTileKernel {
     // ciphertext, only the rows of set bits.
     static __device__ __forceinline__ void syndrome(const FusedArgs &a, const FusedTile &tl, uint32_t *lds, int lane)
     {
-        // row 0: the zero row read for s_{negative}
-        if (lane < kRowWords) lds[lane] = 0;
         const int chunks = (tl.count + CH - 1) / CH;
         for (int c = 0; c < chunks; c++) {
             const int x0 = c * CH;
@@ -592,6 +620,12 @@ struct FusedTileKernel {
                 for (int r = 0; r < R; r++) acc[x][r] = 0;
 
             if (tl.count > SCALAR_BM && 2 * runs <= n_live) {
+                // entry 0 of both four-row tables
+                {
+                    const uint32_t zero[R] = {};
+                    tab_put(lds, 0, lane, zero);
+                    tab_put(lds, 16, lane, zero);
+                }
                 for (uint64_t rem = __ballot(live); rem;) {
                     const int key = __builtin_amdgcn_readlane(my_key, __builtin_ctzll(rem));
                     const bool in_run = my_key == key;
@@ -612,14 +646,14 @@ struct FusedTileKernel {
                         for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
                         const int g = 4 * w;
                         if (g + 1 < NGROUP) load_group(rows, g + 1, lane, vb);
-                        add_group<0>(va, cw, acc);
+                        add_group<0>(va, cw, acc, lds, lane);
                         if (g + 2 < NGROUP) load_group(rows, g + 2, lane, va);
-                        if (g + 1 < NGROUP) add_group<8>(vb, cw, acc);
+                        if (g + 1 < NGROUP) add_group<8>(vb, cw, acc, lds, lane);
                         if (g + 2 >= NGROUP) break;
                         load_group(rows, g + 3, lane, vb);
-                        add_group<16>(va, cw, acc);
+                        add_group<16>(va, cw, acc, lds, lane);
                         if (g + 4 < NGROUP) load_group(rows, g + 4, lane, va);
-                        add_group<24>(vb, cw, acc);
+                        add_group<24>(vb, cw, acc, lds, lane);
                     }
                 }
             } else {
@@ -660,6 +694,8 @@ struct FusedTileKernel {
                 }
             }
 
+            // row 0: the zero row read for s_{negative} (after the four-row tables are done with)
+            if (lane < kRowWords) lds[lane] = 0;
             // transpose: plane (j, k) gets bit k of s_j of chunk ciphertext x at bit x0 + x
 #pragma unroll
             for (int r = 0; r < R; r++) {
