@@ -739,6 +739,119 @@ struct FusedTileKernel {
         return and_xor(shift_up1_zero(v), pm, top);
     }
 
+    // Iteration n of the bit-sliced recurrence.  FULL: every lane block takes part and the
+    // transposed top coefficient is live (n >= NFULL, a public bound), so the step is one
+    // straight-line block the scheduler can interleave; otherwise blocks join by the bounds
+    // above and the top coefficient is still zero.
+    static constexpr int NFULL = TOPX ? T - 2 : 64 * (NBV - 1) - 1;
+    static_assert(NFULL >= 0, "full iterations start at a valid index");
+
+    template <bool FULL>
+    static __device__ __forceinline__ void bm_step(int n, const uint4 *rows4, int lane, uint32_t (&C)[NBL][M],
+                                                   uint32_t (&B)[NBV][M], uint32_t (&bb)[M], uint32_t (&Lp)[9],
+                                                   uint32_t &ct, uint32_t &bt, uint32_t keep_last, uint32_t pm)
+    {
+        // discrepancy d = sum_i C_i s_{n-i} (rows hold s_{j} at j+1, row 0 is zero)
+        uint32_t p[2 * M - 1];
+#pragma unroll
+        for (int s2 = 0; s2 < 2 * M - 1; s2++) p[s2] = 0;
+#pragma unroll
+        for (int b = 0; b < NBV; b++) {
+            if (!FULL && 64 * b > n + 1) continue;
+            const int i = lane + 64 * b;
+            const int row = (n - i + 1) > 0 ? (n - i + 1) : 0;
+            uint32_t sv[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 w = rows4[row * 4 + q];
+                sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
+            }
+            bs_mac_wide<M>(C[b], sv, p);
+        }
+        // + C_t s_{n-t} (TOPX): s_{n-t} (row n-t+1; row 0, zero, while n < t) as wave-uniform
+        // planes, Horner over them
+        [[maybe_unused]] uint32_t top = 0;
+        if constexpr (TOPX && FULL) {
+            const int row = n - T + 1 > 0 ? n - T + 1 : 0;
+            uint32_t sv[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 w = rows4[row * 4 + q];
+                sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
+            }
+            top = and_xor(0u, ct, uni(sv[M - 1]));
+#pragma unroll
+            for (int a = M - 2; a >= 0; a--) top = and_xor(tx_mulx(top, pm), ct, uni(sv[a]));
+        }
+        uint32_t part[M], d[M];
+        bs_fold<M>(p, part);
+#pragma unroll
+        for (int k = 0; k < M; k++) d[k] = wave_xor_all(part[k]);
+        if constexpr (TOPX && FULL) {
+#pragma unroll
+            for (int k = 0; k < M; k++) d[k] ^= (uint32_t)__builtin_amdgcn_readlane((int)top, k);
+        }
+        uint32_t mne = 0;
+#pragma unroll
+        for (int k = 0; k < M; k++) mne |= d[k];
+        const uint32_t mle = mne & bs_le_const(Lp, (uint32_t)n >> 1);
+
+        // C' = b*C + d*B ; B' = mle ? C : B ; then B <- x*B
+        if constexpr (TOPX && FULL) {
+            // C_t' = b C_t + d B_t (B_t' comes from the shift below)
+            uint32_t acc = and_xor(and_xor(0u, ct, bb[M - 1]), bt, d[M - 1]);
+#pragma unroll
+            for (int a = M - 2; a >= 0; a--) acc = and_xor(and_xor(tx_mulx(acc, pm), ct, bb[a]), bt, d[a]);
+            ct = acc;
+        }
+#pragma unroll
+        for (int b = NBV - 1; b >= 0; b--) {
+            if (!FULL && 64 * b > n + 1) continue;
+            uint32_t cn[M];
+            bs_mul_add2<M>(bb, C[b], d, B[b], cn);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                B[b][k] = __builtin_amdgcn_bitop3_b32(mle, C[b][k], B[b][k], 0xCA);  // mle ? C : B
+                C[b][k] = cn[k];
+            }
+        }
+        // the shift moves coefficient t-1 of the selected B into B_t (TOPX); the old B_t
+        // moves past t and is dropped
+        if constexpr (TOPX && FULL) {
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)B[NBV - 1][k], 63);
+                bt = lane == k ? c : bt;
+            }
+        }
+        // shift the highest block first so each carry is read before its source moves; B has
+        // degree <= n+1 here, so block b can receive a carry once 64b <= n+2
+#pragma unroll
+        for (int b = NBV - 1; b >= 0; b--) {
+            if (!FULL && 64 * b > n + 2) continue;
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                if (b == 0) {
+                    B[b][k] = shift_up1_zero(B[b][k]);
+                } else {
+                    const uint32_t carry = (uint32_t)__builtin_amdgcn_readlane((int)B[b > 0 ? b - 1 : 0][k], 63);
+                    B[b][k] = shift_up1(B[b][k], carry);
+                }
+            }
+        }
+        if constexpr (!TOPX) {
+#pragma unroll
+            for (int k = 0; k < M; k++) B[NBV - 1][k] &= keep_last;
+        }
+        // L, b
+        uint32_t Ln[9];
+        bs_const_sub((uint32_t)n + 1, Lp, Ln);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Lp[q] = (Ln[q] & mle) | (Lp[q] & ~mle);
+#pragma unroll
+        for (int k = 0; k < M; k++) bb[k] = (d[k] & mle) | (bb[k] & ~mle);
+    }
+
     static __device__ __forceinline__ void berlekamp_massey(uint32_t *lds, int lane, uint32_t (&C)[NBL][M])
     {
         uint32_t B[NBV][M];
@@ -756,115 +869,15 @@ struct FusedTileKernel {
         for (int q = 0; q < 9; q++) Lp[q] = 0;
 
         // coefficients above T (last block) are dropped like the reference's fixed t+1 array
-        [[maybe_unused]] const uint32_t keep_last = (lane + 64 * (NBV - 1) <= T) ? ~0u : 0u;
-        [[maybe_unused]] const uint32_t pm = (lane < M && ((Modulus<M>::low >> lane) & 1u)) ? ~0u : 0u;
-        [[maybe_unused]] uint32_t ct = 0, bt = 0;  // TOPX: planes of C_t and B_t, transposed
+        const uint32_t keep_last = (lane + 64 * (NBV - 1) <= T) ? ~0u : 0u;
+        const uint32_t pm = (lane < M && ((Modulus<M>::low >> lane) & 1u)) ? ~0u : 0u;
+        uint32_t ct = 0, bt = 0;  // TOPX: planes of C_t and B_t, transposed
         const uint4 *rows4 = reinterpret_cast<const uint4 *>(lds);
 
 #pragma unroll 1
-        for (int n = 0; n < 2 * T; n++) {
-            // discrepancy d = sum_i C_i s_{n-i} (rows hold s_{j} at j+1, row 0 is zero)
-            uint32_t p[2 * M - 1];
-#pragma unroll
-            for (int s2 = 0; s2 < 2 * M - 1; s2++) p[s2] = 0;
-#pragma unroll
-            for (int b = 0; b < NBV; b++) {
-                if (64 * b > n + 1) continue;
-                const int i = lane + 64 * b;
-                const int row = (n - i + 1) > 0 ? (n - i + 1) : 0;
-                uint32_t sv[16];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint4 w = rows4[row * 4 + q];
-                    sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
-                }
-                bs_mac_wide<M>(C[b], sv, p);
-            }
-            uint32_t part[M], d[M];
-            bs_fold<M>(p, part);
-#pragma unroll
-            for (int k = 0; k < M; k++) d[k] = wave_xor_all(part[k]);
-            if constexpr (TOPX) {
-                if (n >= T) {
-                    // + C_t s_{n-t}: s_{n-t} (row n-t+1) as wave-uniform planes, Horner over them
-                    const int row = n - T + 1;
-                    uint32_t sv[16];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const uint4 w = rows4[row * 4 + q];
-                        sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
-                    }
-                    uint32_t acc = and_xor(0u, ct, uni(sv[M - 1]));
-#pragma unroll
-                    for (int a = M - 2; a >= 0; a--) acc = and_xor(tx_mulx(acc, pm), ct, uni(sv[a]));
-#pragma unroll
-                    for (int k = 0; k < M; k++) d[k] ^= (uint32_t)__builtin_amdgcn_readlane((int)acc, k);
-                }
-            }
-            uint32_t mne = 0;
-#pragma unroll
-            for (int k = 0; k < M; k++) mne |= d[k];
-            const uint32_t mle = mne & bs_le_const(Lp, (uint32_t)n >> 1);
-
-            // C' = b*C + d*B ; B' = mle ? C : B ; then B <- x*B
-            if constexpr (TOPX) {
-                if (n + 1 >= T) {
-                    // C_t' = b C_t + d B_t (B_t' comes from the shift below)
-                    uint32_t acc = and_xor(and_xor(0u, ct, bb[M - 1]), bt, d[M - 1]);
-#pragma unroll
-                    for (int a = M - 2; a >= 0; a--) acc = and_xor(and_xor(tx_mulx(acc, pm), ct, bb[a]), bt, d[a]);
-                    ct = acc;
-                }
-            }
-#pragma unroll
-            for (int b = NBV - 1; b >= 0; b--) {
-                if (64 * b > n + 1) continue;
-                uint32_t cn[M];
-                bs_mul_add2<M>(bb, C[b], d, B[b], cn);
-#pragma unroll
-                for (int k = 0; k < M; k++) {
-                    B[b][k] = __builtin_amdgcn_bitop3_b32(mle, C[b][k], B[b][k], 0xCA);  // mle ? C : B
-                    C[b][k] = cn[k];
-                }
-            }
-            // the shift moves coefficient t-1 of the selected B into B_t (TOPX); the old B_t
-            // moves past t and is dropped
-            if constexpr (TOPX) {
-                if (n + 2 >= T) {
-#pragma unroll
-                    for (int k = 0; k < M; k++) {
-                        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)B[NBV - 1][k], 63);
-                        bt = lane == k ? c : bt;
-                    }
-                }
-            }
-            // shift the highest block first so each carry is read before its source moves; B has
-            // degree <= n+1 here, so block b can receive a carry once 64b <= n+2
-#pragma unroll
-            for (int b = NBV - 1; b >= 0; b--) {
-                if (64 * b > n + 2) continue;
-#pragma unroll
-                for (int k = 0; k < M; k++) {
-                    if (b == 0) {
-                        B[b][k] = shift_up1_zero(B[b][k]);
-                    } else {
-                        const uint32_t carry = (uint32_t)__builtin_amdgcn_readlane((int)B[b > 0 ? b - 1 : 0][k], 63);
-                        B[b][k] = shift_up1(B[b][k], carry);
-                    }
-                }
-            }
-            if constexpr (!TOPX) {
-#pragma unroll
-                for (int k = 0; k < M; k++) B[NBV - 1][k] &= keep_last;
-            }
-            // L, b
-            uint32_t Ln[9];
-            bs_const_sub((uint32_t)n + 1, Lp, Ln);
-#pragma unroll
-            for (int q = 0; q < 9; q++) Lp[q] = (Ln[q] & mle) | (Lp[q] & ~mle);
-#pragma unroll
-            for (int k = 0; k < M; k++) bb[k] = (d[k] & mle) | (bb[k] & ~mle);
-        }
+        for (int n = 0; n < NFULL; n++) bm_step<false>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+#pragma unroll 1
+        for (int n = NFULL; n < 2 * T; n++) bm_step<true>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
         if constexpr (TOPX) {
             // C_t into lane 0 of the last block
 #pragma unroll
