@@ -746,18 +746,33 @@ struct FusedTileKernel {
     static constexpr int NFULL = TOPX ? T - 2 : 64 * (NBV - 1) - 1;
     static_assert(NFULL >= 0, "full iterations start at a valid index");
 
-    template <bool FULL>
+    // Block b takes part in the products of iteration n once 64b <= n+1 and in the shift once
+    // 64b <= n+2; inside a phase [LO, HI) of the loop these are decided at compile time where
+    // the phase bounds allow.
+    template <int LO, int HI>
+    static __device__ __forceinline__ bool mac_on(int b, int n)
+    {
+        return 64 * b <= LO + 1 ? true : (64 * b > HI ? false : 64 * b <= n + 1);
+    }
+    template <int LO, int HI>
+    static __device__ __forceinline__ bool shift_on(int b, int n)
+    {
+        return 64 * b <= LO + 2 ? true : (64 * b > HI + 1 ? false : 64 * b <= n + 2);
+    }
+
+    template <int LO, int HI>
     static __device__ __forceinline__ void bm_step(int n, const uint4 *rows4, int lane, uint32_t (&C)[NBL][M],
                                                    uint32_t (&B)[NBV][M], uint32_t (&bb)[M], uint32_t (&Lp)[9],
                                                    uint32_t &ct, uint32_t &bt, uint32_t keep_last, uint32_t pm)
     {
+        constexpr bool FULL = LO >= NFULL;
         // discrepancy d = sum_i C_i s_{n-i} (rows hold s_{j} at j+1, row 0 is zero)
         uint32_t p[2 * M - 1];
 #pragma unroll
         for (int s2 = 0; s2 < 2 * M - 1; s2++) p[s2] = 0;
 #pragma unroll
         for (int b = 0; b < NBV; b++) {
-            if (!FULL && 64 * b > n + 1) continue;
+            if (!mac_on<LO, HI>(b, n)) continue;
             const int i = lane + 64 * b;
             const int row = (n - i + 1) > 0 ? (n - i + 1) : 0;
             uint32_t sv[16];
@@ -806,7 +821,7 @@ struct FusedTileKernel {
         }
 #pragma unroll
         for (int b = NBV - 1; b >= 0; b--) {
-            if (!FULL && 64 * b > n + 1) continue;
+            if (!mac_on<LO, HI>(b, n)) continue;
             uint32_t cn[M];
             bs_mul_add2<M>(bb, C[b], d, B[b], cn);
 #pragma unroll
@@ -828,7 +843,7 @@ struct FusedTileKernel {
         // degree <= n+1 here, so block b can receive a carry once 64b <= n+2
 #pragma unroll
         for (int b = NBV - 1; b >= 0; b--) {
-            if (!FULL && 64 * b > n + 2) continue;
+            if (!shift_on<LO, HI>(b, n)) continue;
 #pragma unroll
             for (int k = 0; k < M; k++) {
                 if (b == 0) {
@@ -874,10 +889,19 @@ struct FusedTileKernel {
         uint32_t ct = 0, bt = 0;  // TOPX: planes of C_t and B_t, transposed
         const uint4 *rows4 = reinterpret_cast<const uint4 *>(lds);
 
+        // phases between the indices where a block joins the shift (64b-2) or the products
+        // (64b-1); blocks 1 and 2 at most, as t <= 128
+        static_assert(NBV <= 2, "phase split covers two lane blocks");
+        constexpr int P1 = NBV > 1 ? 62 : NFULL, P2 = NBV > 1 ? 63 : NFULL;
+        static_assert(P1 <= P2 && P2 <= NFULL, "phases in order");
 #pragma unroll 1
-        for (int n = 0; n < NFULL; n++) bm_step<false>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+        for (int n = 0; n < P1; n++) bm_step<0, P1>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
 #pragma unroll 1
-        for (int n = NFULL; n < 2 * T; n++) bm_step<true>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+        for (int n = P1; n < P2; n++) bm_step<P1, P2>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+#pragma unroll 1
+        for (int n = P2; n < NFULL; n++) bm_step<P2, NFULL>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+#pragma unroll 1
+        for (int n = NFULL; n < 2 * T; n++) bm_step<NFULL, 2 * T>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
         if constexpr (TOPX) {
             // C_t into lane 0 of the last block
 #pragma unroll
