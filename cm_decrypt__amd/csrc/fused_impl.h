@@ -920,6 +920,47 @@ struct FusedTileKernel {
     static constexpr int SCALAR_BM = 3;
     static constexpr int NQ = (2 * T + 63) / 64;  // syndromes per lane
 
+    // One iteration of the per-ciphertext recurrence, inside a phase [LO, HI) of n; the
+    // selects on mle (secret) are branch-free.
+    template <int LO, int HI>
+    static __device__ __forceinline__ void scalar_bm_step(int n, const uint16_t *Sx, int lane, uint32_t keep_last,
+                                                          uint32_t (&c)[NBL], uint32_t (&bv)[NBL], uint32_t &bb,
+                                                          uint32_t &L, uint32_t (&colb)[M])
+    {
+        // the lanes' products are summed unreduced; the sum reduces once
+        uint32_t part = 0;
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+            if (!mac_on<LO, HI>(b, n)) continue;
+            const int j = n - (lane + 64 * b);
+            const uint32_t sv = j >= 0 ? (uint32_t)Sx[j] : 0u;
+            part ^= gf_clmul<M>(c[b], sv);
+        }
+        const uint32_t d = gf_reduce<M>(uni(wave_xor_all(part)));
+        const bool mle = d != 0 && 2 * L <= (uint32_t)n;
+        uint32_t cold[M];
+        gf_mul_cols<M>(d, cold);
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+            if (!mac_on<LO, HI>(b, n)) continue;
+            const uint32_t cn = gf_mul_by_cols<M>(colb, c[b]) ^ gf_mul_by_cols<M>(cold, bv[b]);
+            bv[b] = mle ? c[b] : bv[b];
+            c[b] = cn;
+        }
+#pragma unroll
+        for (int b = NBL - 1; b >= 0; b--) {
+            if (!shift_on<LO, HI>(b, n)) continue;
+            const uint32_t carry = b == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)bv[b > 0 ? b - 1 : 0], 63);
+            bv[b] = shift_up1(bv[b], carry);
+        }
+        bv[NBL - 1] &= keep_last;
+        // wave-uniform selects (one s_cselect each), no branch on the secret mle
+        L = mle ? (uint32_t)n + 1 - L : L;
+        bb = mle ? d : bb;
+#pragma unroll
+        for (int k = 0; k < M; k++) colb[k] = mle ? cold[k] : colb[k];
+    }
+
     static __device__ __forceinline__ void berlekamp_massey_scalar(uint32_t *lds, int lane, int count,
                                                                    uint32_t (&C)[NBL][M])
     {
@@ -966,43 +1007,10 @@ struct FusedTileKernel {
             uint32_t bb = 1, L = 0;
             uint32_t colb[M];  // multiplication table of bb, renewed when bb changes
             gf_mul_cols<M>(bb, colb);
+            // one loop with run-time block gating: this path serves latency-bound launches of
+            // one to three ciphertexts, where a smaller body matters more than branch-free steps
 #pragma unroll 1
-            for (int n = 0; n < 2 * T; n++) {
-                // the lanes' products are summed unreduced; the sum reduces once
-                uint32_t part = 0;
-#pragma unroll
-                for (int b = 0; b < NBL; b++) {
-                    if (64 * b > n + 1) continue;
-                    const int j = n - (lane + 64 * b);
-                    const uint32_t sv = j >= 0 ? (uint32_t)Sx[j] : 0u;
-                    part ^= gf_clmul<M>(c[b], sv);
-                }
-                const uint32_t d = gf_reduce<M>(uni(wave_xor_all(part)));
-                const bool mle = d != 0 && 2 * L <= (uint32_t)n;
-                uint32_t cold[M];
-                gf_mul_cols<M>(d, cold);
-#pragma unroll
-                for (int b = 0; b < NBL; b++) {
-                    if (64 * b > n + 1) continue;
-                    const uint32_t cn = gf_mul_by_cols<M>(colb, c[b]) ^ gf_mul_by_cols<M>(cold, bv[b]);
-                    bv[b] = mle ? c[b] : bv[b];
-                    c[b] = cn;
-                }
-#pragma unroll
-                for (int b = NBL - 1; b >= 0; b--) {
-                    if (64 * b > n + 2) continue;
-                    const uint32_t carry =
-                        b == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)bv[b > 0 ? b - 1 : 0], 63);
-                    bv[b] = shift_up1(bv[b], carry);
-                }
-                bv[NBL - 1] &= keep_last;
-                if (mle) {
-                    L = (uint32_t)n + 1 - L;
-                    bb = d;
-#pragma unroll
-                    for (int k = 0; k < M; k++) colb[k] = cold[k];
-                }
-            }
+            for (int n = 0; n < 2 * T; n++) scalar_bm_step<0, 2 * T>(n, Sx, lane, keep_last, c, bv, bb, L, colb);
 #pragma unroll
             for (int b = 0; b < NBL; b++)
 #pragma unroll

@@ -38,8 +38,16 @@ hipError_t launch_fused(const FusedArgs &a, const FusedSchedule &s, int n_cu, hi
     if (a.n_tiles <= 0) return hipSuccess;
     // more than one wavefront per SIMD needs the 256-register build
     const bool occ2 = s.waves_per_group * s.groups_per_cu > 4;
-    if (s.root_alg == 1) return occ2 ? launch_fused_fft_occ2(a, s, n_cu, st) : launch_fused_fft_occ1(a, s, n_cu, st);
-    return occ2 ? launch_fused_horner_occ2(a, s, n_cu, st) : launch_fused_horner_occ1(a, s, n_cu, st);
+    // Fewer tiles than resident wavefronts (small and serving launches): one wavefront per
+    // workgroup, so the tiles land on as many CUs as there are tiles instead of sharing a few
+    // CUs' scalar units, LDS and caches; the build and the wavefronts per CU stay the same.
+    FusedSchedule t = s;
+    if ((long)a.n_tiles < (long)n_cu * s.waves_per_group * s.groups_per_cu) {
+        t.groups_per_cu = s.waves_per_group * s.groups_per_cu;
+        t.waves_per_group = 1;
+    }
+    if (t.root_alg == 1) return occ2 ? launch_fused_fft_occ2(a, t, n_cu, st) : launch_fused_fft_occ1(a, t, n_cu, st);
+    return occ2 ? launch_fused_horner_occ2(a, t, n_cu, st) : launch_fused_horner_occ1(a, t, n_cu, st);
 }
 
 }  // namespace cmd

@@ -203,6 +203,26 @@ def test_fused_schedules_agree(ctx, keys, alg, tile, wpg, gpc):
     assert np.array_equal(fitted, want), "narrowed tiles differ"
 
 
+@pytest.mark.parametrize("tile,wpg,gpc", [(1, 4, 1), (1, 2, 2), (2, 4, 2)])
+def test_multiwave_workgroups_full_grid(ctx, keys, tile, wpg, gpc):
+    """Launches with at least as many tiles as resident wavefronts keep the schedule's
+    wavefronts per workgroup (smaller ones run one per workgroup): every wavefront of a
+    multi-wavefront group, each on its own slice of LDS, recovers the generated errors."""
+    pid = 348864
+    sk, kid = keys[pid]
+    n = tile * 256 * wpg * gpc + 77  # more tiles than the grid has wavefronts (256 CUs)
+    ct, e = cm.gen_encrypt(pid, sk, 901 + tile, n)
+    saved = ctx.schedule(pid)
+    try:
+        ctx.set_tile_fit(False)
+        ctx.set_schedule(pid, tile_cts=tile, waves_per_group=wpg, groups_per_cu=gpc, root_alg=1)
+        got = ctx.decrypt([kid] * n, ct, mode=cm.MODE_FUSED)
+    finally:
+        ctx.set_tile_fit(True)
+        ctx.set_schedule(pid, **saved)
+    assert np.array_equal(got[:, :e.shape[1]], e)
+
+
 @pytest.mark.parametrize("pid,K", [(348864, 63), (8192128, 63), (8192128, 127), (6688128, 127)])
 def test_late_first_syndrome(ctx, keys, pid, K):
     """Syndromes zero below index K put the first nonzero discrepancy on a 64-coefficient
