@@ -151,6 +151,31 @@ def test_key_runs_inside_tiles(ctx, pid):
             ctx.unload_key(k)
 
 
+@pytest.mark.parametrize("pid", [348864, 460896])
+def test_random_key_runs_fitted_tiles(ctx, pid):
+    """Random run lengths over twelve keys, valid and random ciphertexts, shuffled, with the
+    planner's own tile widths (fitted to the batch): every row against the oracle."""
+    rng = np.random.default_rng(pid % 1000)
+    sks = cm.gen_keys(pid, 57, 12, threads=8)
+    kids = [ctx.load_key(pid, s) for s in sks]
+    try:
+        rows, kk, want = [], [], []
+        for k in range(12):
+            n = int(rng.integers(1, 60))
+            ct, _ = inputs(pid, sks[k], n, int(rng.integers(0, 3)), 700 + k)
+            want.append(orc.decrypt(pid, sks[k], ct))
+            rows.append(ct)
+            kk += [kids[k]] * len(ct)
+        ct = np.concatenate(rows)
+        want = np.concatenate(want)
+        order = rng.permutation(len(ct))
+        got = ctx.decrypt(np.array(kk, np.int32)[order], ct[order], mode=cm.MODE_FUSED)
+        assert np.array_equal(got, want[order])
+    finally:
+        for k in kids:
+            ctx.unload_key(k)
+
+
 @pytest.mark.parametrize("pid", PIDS)
 def test_weight_below_t(ctx, keys, pid):
     """w < t leaves a factor x^(t-w) in the reversed locator: a support element equal to 0
