@@ -7,21 +7,28 @@
 // the list is exhausted (every wave exits when the counter passes n_tiles).
 //
 //  1. syndrome (CPU/param/param6960119/src/util.c:102-114): lane l owns table columns
-//     2c, 2c+1 for c = l + 64r; each table row is streamed once per 16 ciphertexts and
-//     masked with wave-uniform ciphertext bits.  The 2t syndromes are transposed into LDS
-//     bit planes: word [j][k] holds bit k of s_j for every ciphertext of the tile (bit x =
-//     tile slot x).
+//     2c, 2c+1 for c = l + 64r; each run of ciphertexts of one key streams that key's table
+//     rows once for the tile, four rows at a time through per-wavefront LDS tables of their
+//     16 XOR combinations, each ciphertext adding the entry its nibble selects (tiles of many
+//     keys, and tiles of at most three ciphertexts, read only the rows of set bits instead).
+//     The 2t syndromes are transposed into LDS bit planes: word [j][k] holds bit k of s_j
+//     for every ciphertext of the tile (bit x = tile slot x).
 //  2. Berlekamp-Massey (CPU/src/common/bm.c:12-68), bit-sliced across the tile: lane l holds
-//     planes of coefficients l + 64b of C and B; discrepancies reduce over lanes with DPP,
-//     the per-ciphertext discrepancy, masks mne/mle, L and b live in scalar registers.  Inversion-free
-//     update C' = b*C + d*B keeps every iterate a nonzero multiple of the reference's C, so
-//     the zeros of the locator (all that the error vector depends on) are the reference's.
-//  3. root search (root_13.c:233-240, decrypt.c:43-53): for each ciphertext, lane l evaluates
-//     the reversed locator by Horner at 32 support points at once (bit-sliced L planes of the
-//     key); a position is an error iff all m planes are zero there, which yields the packed
-//     error-vector word directly.
-// No step indexes memory or branches on secret data; the control flow depends only on
-// public sizes and the iteration counter.
+//     planes of coefficients l + 64b of C and B (coefficient t, when t is a multiple of 64,
+//     transposed over lanes instead); discrepancies reduce over lanes with DPP, the
+//     per-ciphertext discrepancy, masks mne/mle, L and b live in scalar registers.
+//     Inversion-free update C' = b*C + d*B keeps every iterate a nonzero multiple of the
+//     reference's C, so the zeros of the locator (all that the error vector depends on) are
+//     the reference's.  Tiles of at most three ciphertexts run the recurrence per ciphertext
+//     with scalar field elements.
+//  3. root search (root_13.c:233-240, decrypt.c:43-53), per ciphertext: either Horner at 32
+//     support points per lane (bit-sliced L planes of the key), or an additive FFT over all
+//     2^m field elements followed by the key's Benes network, which moves each root indicator
+//     to its support position; a position is an error iff all m planes are zero there,
+//     which yields the packed error-vector word directly.
+// No step indexes memory or branches on secret data (the public ciphertext bits select table
+// rows and LDS entries); the control flow depends only on public sizes, the ciphertext and
+// the iteration counter.
 #pragma once
 
 #include <hip/hip_runtime.h>
