@@ -8,7 +8,8 @@
 //  * butterfly at depth d for the point index c' of the lower half:
 //      G(c') = sum_j c'_j gamma_j,  w0 = u + G v,  w1 = w0 + v.
 //    With c' = 32 i + b (word offset i in the half, bit b in the word), G splits into
-//    low(b) (bits of b) and hi(i) (bits of i); lowpat stores low(b) bit-sliced over b.
+//    low(b) (bits of b) and hi(i) (bits of i); lowpat stores low(b) bit-sliced over b, gpl the
+//    sum of both as whole bit-sliced planes per word offset.
 #include <cstring>
 
 #include "host_gf.h"
@@ -51,6 +52,9 @@ void build_fft_tab(int m, int l, FftTab *out)
                 if ((i >> (j - 5)) & 1) hv ^= gam[j];
             out->hi[d][i] = hv;
         }
+        for (int i = 0; i < 128; i++)
+            for (int q = 0; q < m; q++)
+                out->gpl[d][i][q] = out->lowpat[d][q] ^ (((out->hi[d][i] >> q) & 1u) ? ~0u : 0u);
         for (int j = 0; j < k - 1; j++) basis[j] = F.mul(gam[j], gam[j]) ^ gam[j];
         k--;
     }

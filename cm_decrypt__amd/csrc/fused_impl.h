@@ -384,20 +384,25 @@ struct FftPlan {
         if constexpr (SL >= 0) swap_bits<(SL >= 0 ? SL : 0), (SQ >= 0 ? SQ : 0)>(W, lane);
         constexpr int RB = 1 << SCHED.bq[LI];
         constexpr Layout LAY = SCHED.at[LI];
-        // Reload this level's constants here: hoisted out of the per-ciphertext loop, the
+        // Load this level's multipliers here: hoisted out of the per-ciphertext loop, the
         // multipliers of all levels would not fit in registers and would go to scratch.
         asm volatile("" ::: "memory");
-        uint32_t lowp[M];
-#pragma unroll
-        for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
         const uint32_t wl = lane_part(LAY, opaque(lane), J);
 #pragma unroll
         for (int r = 0; r < WPL; r++) {
             if (r & RB) continue;
-            const uint32_t hv = tab->hi[Dl][wl | reg_part(LAY, r, J)];
+            // the m planes of G for this word, precomputed (three 16-byte loads and one more
+            // for m = 13)
+            const uint4 *gp = reinterpret_cast<const uint4 *>(tab->gpl[Dl][wl | reg_part(LAY, r, J)]);
             uint32_t G[M], t[M];
 #pragma unroll
-            for (int k = 0; k < M; k++) G[k] = lowp[k] ^ bitmask(hv, k);
+            for (int q = 0; q < (M + 3) / 4; q++) {
+                const uint4 g4 = gp[q];
+                const uint32_t gw[4] = {g4.x, g4.y, g4.z, g4.w};
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    if (4 * q + e < M) G[4 * q + e] = gw[e];
+            }
             bs_mul<M>(G, W[r | RB], t);
 #pragma unroll
             for (int k = 0; k < M; k++) {

@@ -27,6 +27,7 @@ struct FftTab {
     uint16_t tw[8][256];      // twist at depth d for position p: lambda_d^(p >> d)
     uint16_t hi[8][128];      // butterfly multiplier, high part, by word offset in the half
     uint32_t lowpat[8][16];   // butterfly multiplier, low part, bit-sliced over the 32 bits of a word
+    uint32_t gpl[8][128][16]; // whole multiplier planes: lowpat ^ (bit q of hi ? ~0 : 0), 16-aligned rows
 };
 constexpr int kFftVariants = 3;  // (m, l) = (12, 7), (13, 7), (13, 8)
 constexpr int fft_variant(int m, int l) { return m == 12 ? 0 : (l == 7 ? 1 : 2); }
