@@ -373,6 +373,22 @@ struct FftPlan {
         }
     }
 
+    // The m bit planes of the butterfly multiplier G of level Dl for word offset i, read
+    // precomputed (lowpat ^ hi) from the table in 16-byte loads.
+    template <int Dl>
+    static __device__ __forceinline__ void load_g(const FftTab *tab, uint32_t i, uint32_t (&G)[M])
+    {
+        const uint4 *gp = reinterpret_cast<const uint4 *>(tab->gpl[Dl][i]);
+#pragma unroll
+        for (int q = 0; q < (M + 3) / 4; q++) {
+            const uint4 g4 = gp[q];
+            const uint32_t gw[4] = {g4.x, g4.y, g4.z, g4.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (4 * q + e < M) G[4 * q + e] = gw[e];
+        }
+    }
+
     // Butterfly level Dl (word-index bit J = M - 6 - Dl): w0 = u + G v, w1 = w0 + v, where
     // G = lowpat (by bit position) + hi (by the low J bits of the lower word's index).
     template <int Dl>
@@ -391,18 +407,8 @@ struct FftPlan {
 #pragma unroll
         for (int r = 0; r < WPL; r++) {
             if (r & RB) continue;
-            // the m planes of G for this word, precomputed (three 16-byte loads and one more
-            // for m = 13)
-            const uint4 *gp = reinterpret_cast<const uint4 *>(tab->gpl[Dl][wl | reg_part(LAY, r, J)]);
             uint32_t G[M], t[M];
-#pragma unroll
-            for (int q = 0; q < (M + 3) / 4; q++) {
-                const uint4 g4 = gp[q];
-                const uint32_t gw[4] = {g4.x, g4.y, g4.z, g4.w};
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-                    if (4 * q + e < M) G[4 * q + e] = gw[e];
-            }
+            load_g<Dl>(tab, wl | reg_part(LAY, r, J), G);
             bs_mul<M>(G, W[r | RB], t);
 #pragma unroll
             for (int k = 0; k < M; k++) {
@@ -480,20 +486,14 @@ struct FftPlan {
         constexpr int RB = 1 << SCHED.bq[LI];
         constexpr Layout LAY = SCHED.at[LI];
         asm volatile("" ::: "memory");
-        uint32_t lowp[M];
-#pragma unroll
-        for (int k = 0; k < M; k++) lowp[k] = tab->lowpat[Dl][k];
         const uint32_t wl = lane_part(LAY, opaque(lane), J);
 #pragma unroll
         for (int r = 0; r < WPL; r++) {
             if (r & RB) continue;
-            const uint32_t hv = tab->hi[Dl][wl | reg_part(LAY, r, J)];
             uint32_t G[M], t[M];
+            load_g<Dl>(tab, wl | reg_part(LAY, r, J), G);
 #pragma unroll
-            for (int k = 0; k < M; k++) {
-                G[k] = lowp[k] ^ bitmask(hv, k);
-                W[r][k] ^= W[r | RB][k];
-            }
+            for (int k = 0; k < M; k++) W[r][k] ^= W[r | RB][k];
             bs_mul<M>(G, W[r], t);
 #pragma unroll
             for (int k = 0; k < M; k++) W[r | RB][k] ^= t[k];
