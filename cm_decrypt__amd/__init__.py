@@ -99,8 +99,10 @@ class ServerConfig(_c.Structure):
 
 class ServerStats(_c.Structure):
     _fields_ = [("submitted", _c.c_uint64), ("completed", _c.c_uint64), ("rejected", _c.c_uint64),
-                ("launches", _c.c_uint64), ("mean_batch", _c.c_double), ("p50_us", _c.c_double),
-                ("p99_us", _c.c_double), ("max_us", _c.c_double), ("busy_s", _c.c_double)]
+                ("launches", _c.c_uint64), ("failed", _c.c_uint64), ("mean_batch", _c.c_double),
+                ("p50_us", _c.c_double), ("p99_us", _c.c_double), ("p999_us", _c.c_double),
+                ("max_us", _c.c_double), ("queue_p99_us", _c.c_double), ("queue_max_us", _c.c_double),
+                ("device_p99_us", _c.c_double), ("device_max_us", _c.c_double), ("busy_s", _c.c_double)]
 
     def as_dict(self):
         return {n: (float(getattr(self, n)) if t is _c.c_double else int(getattr(self, n)))
@@ -131,6 +133,7 @@ _sig("cmd_controlbits", _I, _I32, _P, _P)
 _sig("cmd_ctx_create", _I, _I32, _U64, _c.POINTER(_P))
 _sig("cmd_ctx_destroy", _I, _P)
 _sig("cmd_ctx_memory", _I, _P, _c.POINTER(_U64), _c.POINTER(_U64))
+_sig("cmd_ctx_set_budget", _I, _P, _U64)
 _sig("cmd_key_load", _I, _P, _I32, _P, _SZ, _c.POINTER(_I32))
 _sig("cmd_key_unload", _I, _P, _I32)
 _sig("cmd_key_device_bytes", _I, _I32, _c.POINTER(_U64))
@@ -158,9 +161,12 @@ _sig("cmd_schedule_load", _I, _P, _c.c_char_p)
 _sig("cmd_server_create", _I, _P, _c.POINTER(ServerConfig), _c.POINTER(_P))
 _sig("cmd_server_submit", _I, _P, _I32, _P, _c.POINTER(_U64))
 _sig("cmd_server_submit_batch", _I, _P, _SZ, _P, _P, _SZ, _P, _c.POINTER(_SZ))
+_sig("cmd_server_submit_batch_mode", _I, _P, _I32, _SZ, _P, _P, _SZ, _P, _c.POINTER(_SZ))
+_sig("cmd_server_result_bytes", _I, _P, _U64, _c.POINTER(_SZ))
 _sig("cmd_server_wait_batch", _I, _P, _SZ, _P, _P, _SZ, _c.c_double)
-_sig("cmd_server_poll", _I, _P, _U64, _P)
-_sig("cmd_server_wait", _I, _P, _U64, _P, _c.c_double)
+_sig("cmd_server_poll", _I, _P, _U64, _P, _SZ)
+_sig("cmd_server_wait", _I, _P, _U64, _P, _SZ, _c.c_double)
+_sig("cmd_server_latencies", _I, _P, _P, _P, _SZ, _c.POINTER(_SZ))
 _sig("cmd_server_stats_get", _I, _P, _c.POINTER(ServerStats))
 _sig("cmd_server_reset_stats", _I, _P)
 _sig("cmd_server_destroy", _I, _P)
@@ -175,6 +181,12 @@ lib = _lib
 def _check(rc: int, where: str):
     if rc != CMD_OK:
         raise CmdError(rc, where)
+
+
+def _check_rows(buf: np.ndarray, count: int, stride: int, name: str):
+    """The library reads `count` rows `stride` bytes apart: refuse a buffer that is smaller."""
+    if count and buf.nbytes < count * stride:
+        raise ValueError(f"{name}: {buf.nbytes} bytes for {count} rows of {stride}")
 
 
 def _ptr(a) -> Optional[int]:
@@ -344,6 +356,10 @@ class Context:
         _check(_lib.cmd_ctx_memory(self._h, _c.byref(used), _c.byref(budget)), "cmd_ctx_memory")
         return int(used.value), int(budget.value)
 
+    def set_budget(self, hbm_budget_bytes: int):
+        """New HBM budget for later allocations (keys, work buffers, server scratch)."""
+        _check(_lib.cmd_ctx_set_budget(self._h, hbm_budget_bytes), "cmd_ctx_set_budget")
+
     def load_key(self, param_id: int, sk: np.ndarray) -> int:
         sk = np.ascontiguousarray(sk, dtype=np.uint8)
         kid = _I32(-1)
@@ -385,6 +401,7 @@ class Context:
             # itself rejects unknown keys with CMD_ERR_KEY
             e_stride = max((params(self.key_param[int(k)]).e_bytes if int(k) in self.key_param else 1024)
                            for k in np.unique(keys)) if count else 0
+        _check_rows(ct, count, ct_stride, "ct")
         if out is None:
             e = np.zeros((count, e_stride), np.uint8)
         else:
@@ -434,6 +451,7 @@ class Context:
         count = keys.size
         if ct_stride is None:
             ct_stride = ct.shape[1] if ct.ndim == 2 else ct.size // max(count, 1)
+        _check_rows(ct, count, ct_stride, "ct")
         out = np.zeros((count, KEM_KEY_BYTES), np.uint8)
         _check(_lib.cmd_kem_dec(self._h, count, _ptr(keys), _ptr(ct), ct_stride, _ptr(out), KEM_KEY_BYTES),
                "cmd_kem_dec")
@@ -562,25 +580,33 @@ class Server:
         self.ctx = ctx
 
     def submit(self, key_id: int, ct: np.ndarray) -> int:
+        """One decryption request; the ciphertext is the key's ct_bytes (shorter buffers are refused)."""
         ct = np.ascontiguousarray(ct, dtype=np.uint8)
+        pid = self.ctx.key_param.get(int(key_id))
+        if pid is not None and ct.nbytes < params(pid).ct_bytes:
+            raise ValueError(f"ciphertext of {ct.nbytes} bytes, {params(pid).ct_bytes} needed")
         rid = _U64(0)
-        _check(_lib.cmd_server_submit(self._h, key_id, _ptr(ct), _c.byref(rid)), "cmd_server_submit")
+        _check(_lib.cmd_server_submit_batch(self._h, 1, _ptr(np.array([key_id], np.int32)), _ptr(ct), ct.nbytes,
+                                            _c.byref(rid), None), "cmd_server_submit")
         return int(rid.value)
 
-    def submit_batch(self, key_ids, ct: np.ndarray, ct_stride: Optional[int] = None) -> np.ndarray:
-        """Submits len(key_ids) requests (rows of ct); returns their ids."""
+    def submit_batch(self, key_ids, ct: np.ndarray, ct_stride: Optional[int] = None,
+                     mode: int = MODE_FUSED) -> np.ndarray:
+        """Submits len(key_ids) requests (rows of ct) of `mode` (MODE_FUSED: error vectors,
+        MODE_KEM: session keys); returns their ids."""
         keys = np.ascontiguousarray(key_ids, dtype=np.int32)
         ct = np.ascontiguousarray(ct, dtype=np.uint8)
         if ct_stride is None:
             ct_stride = ct.shape[1] if ct.ndim == 2 else ct.size // max(keys.size, 1)
+        _check_rows(ct, keys.size, ct_stride, "ct")
         rids = np.zeros(keys.size, np.uint64)
         acc = _SZ(0)
-        _check(_lib.cmd_server_submit_batch(self._h, keys.size, _ptr(keys), _ptr(ct), ct_stride, _ptr(rids),
-                                            _c.byref(acc)), "cmd_server_submit_batch")
+        _check(_lib.cmd_server_submit_batch_mode(self._h, mode, keys.size, _ptr(keys), _ptr(ct), ct_stride,
+                                                 _ptr(rids), _c.byref(acc)), "cmd_server_submit_batch_mode")
         return rids
 
     def wait_batch(self, rids: np.ndarray, e_stride: int = 1024, timeout_us: float = -1.0) -> np.ndarray:
-        """Error vectors of the requests rids (rows of e_stride bytes)."""
+        """Results of the requests rids (rows of e_stride bytes: error vectors or session keys)."""
         rids = np.ascontiguousarray(rids, dtype=np.uint64)
         e = np.zeros((rids.size, e_stride), np.uint8)
         _check(_lib.cmd_server_wait_batch(self._h, rids.size, _ptr(rids), _ptr(e), e_stride, timeout_us),
@@ -590,12 +616,18 @@ class Server:
     def try_submit(self, key_id: int, ct: np.ndarray):
         ct = np.ascontiguousarray(ct, dtype=np.uint8)
         rid = _U64(0)
-        rc = _lib.cmd_server_submit(self._h, key_id, _ptr(ct), _c.byref(rid))
+        rc = _lib.cmd_server_submit_batch(self._h, 1, _ptr(np.array([key_id], np.int32)), _ptr(ct), ct.nbytes,
+                                          _c.byref(rid), None)
         return rc, int(rid.value)
+
+    def result_bytes(self, req: int) -> int:
+        n = _SZ(0)
+        _check(_lib.cmd_server_result_bytes(self._h, req, _c.byref(n)), "cmd_server_result_bytes")
+        return int(n.value)
 
     def poll(self, req: int, e_bytes: int):
         e = np.zeros(e_bytes, np.uint8)
-        rc = _lib.cmd_server_poll(self._h, req, _ptr(e))
+        rc = _lib.cmd_server_poll(self._h, req, _ptr(e), e.nbytes)
         if rc == CMD_ERR_PENDING:
             return None
         _check(rc, "cmd_server_poll")
@@ -603,8 +635,18 @@ class Server:
 
     def wait(self, req: int, e_bytes: int, timeout_us: float = -1.0) -> np.ndarray:
         e = np.zeros(e_bytes, np.uint8)
-        _check(_lib.cmd_server_wait(self._h, req, _ptr(e), timeout_us), "cmd_server_wait")
+        _check(_lib.cmd_server_wait(self._h, req, _ptr(e), e.nbytes, timeout_us), "cmd_server_wait")
         return e
+
+    def latencies(self):
+        """(total_us, queue_us) per retired request since the last reset_stats()."""
+        n = _SZ(0)
+        _check(_lib.cmd_server_latencies(self._h, None, None, 0, _c.byref(n)), "cmd_server_latencies")
+        tot = np.zeros(n.value, np.float64)
+        que = np.zeros(n.value, np.float64)
+        _check(_lib.cmd_server_latencies(self._h, _ptr(tot), _ptr(que), n.value, _c.byref(n)),
+               "cmd_server_latencies")
+        return tot, que
 
     def stats(self) -> dict:
         st = ServerStats()

@@ -105,6 +105,15 @@ static int upload_key_slot(cmd_ctx *ctx, int32_t id)
     return CMD_OK;
 }
 
+// Waits for the work of the context's own streams (its default stream and the host-buffer
+// pipeline's slots); other streams of the device are left alone.
+static void ctx_quiesce(cmd_ctx *ctx)
+{
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (IoSlot &s : ctx->io)
+        if (s.st) (void)hipStreamSynchronize(s.st);
+}
+
 // ---- batch planning ---------------------------------------------------------------------
 
 void batch_release(cmd_batch *b)
@@ -127,6 +136,11 @@ int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, co
                std::vector<FusedTile> &tiles, std::vector<int32_t> &items, std::vector<int32_t> &item_key,
                int *launch_tile)
 {
+    for (size_t i = 0; i < count; i++) {
+        const int32_t k = key_ids[i];
+        if (k < 0 || (size_t)k >= ctx->keys.size() || !ctx->keys[k].used || ctx->keys[k].pidx < 0)
+            return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)k);
+    }
     std::vector<int32_t> order(count);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
@@ -682,6 +696,8 @@ extern "C" int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx
     CMD_HIP(hipEventCreate(&ctx->ev1));
     for (int i = 0; i < kNumParams; i++) ctx->sched[i] = default_schedule(i);
     CMD_HIP(hipMalloc(&ctx->d_keys, kMaxKeys * sizeof(KeyDev)));
+    // unused slots read as null descriptors (pidx 0 with null tables), never as stale pointers
+    CMD_HIP(hipMemset(ctx->d_keys, 0, kMaxKeys * sizeof(KeyDev)));
     {
         std::vector<FftTab> tabs(kFftVariants);
         build_fft_tab(12, 7, &tabs[0]);
@@ -718,6 +734,14 @@ extern "C" int cmd_ctx_memory(cmd_ctx *ctx, uint64_t *used, uint64_t *budget)
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
     if (used) *used = ctx->used;
     if (budget) *budget = ctx->budget;
+    return CMD_OK;
+}
+
+extern "C" int cmd_ctx_set_budget(cmd_ctx *ctx, uint64_t hbm_budget_bytes)
+{
+    if (!ctx || !hbm_budget_bytes) return CMD_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    ctx->budget = hbm_budget_bytes;
     return CMD_OK;
 }
 
@@ -827,7 +851,9 @@ extern "C" int cmd_key_unload(cmd_ctx *ctx, int32_t key_id)
     if (ctx->keys[key_id].inflight > 0)
         return set_error(CMD_ERR_KEY, "key id %d has %d requests in flight", (int)key_id, ctx->keys[key_id].inflight);
     (void)hipSetDevice(ctx->device);
-    (void)hipDeviceSynchronize();
+    // No batch or server request holds the key (inflight == 0) and every other entry point
+    // waits for its own work; finish what the context's own streams may still run.
+    ctx_quiesce(ctx);
     KeySlot &k = ctx->keys[key_id];
     (void)hipFree(k.mem);
     ctx->used -= k.bytes;
@@ -854,8 +880,13 @@ static int make_batch(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *k
     else
         for (int i = 0; i < kNumParams; i++) b->sched[i] = ctx->sched[i];
     int rc = batch_plan(b.get(), key_ids);
+    if (!rc && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+        b->done = nullptr;
+        rc = set_error(CMD_ERR_HIP, "batch completion event");
+    }
     if (rc) {
         batch_release(b.get());
+        if (b->done) (void)hipEventDestroy(b->done);
         return rc;
     }
     // a captured graph holds the keys' device pointers: keep them loaded while the batch lives
@@ -897,6 +928,10 @@ static int run_batch(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
         int rc = batch_enqueue(b, d_ct, d_e, st);
         if (rc) return rc;
     }
+    if (b->done) {
+        CMD_HIP(hipEventRecord(b->done, st));
+        b->ran = true;
+    }
     if (ctx->timing) {
         CMD_HIP(hipEventRecord(ctx->ev1, st));
         CMD_HIP(hipEventSynchronize(ctx->ev1));
@@ -930,8 +965,10 @@ extern "C" int cmd_batch_destroy(cmd_batch *b)
     if (!b) return CMD_ERR_ARG;
     std::lock_guard<std::recursive_mutex> g(b->ctx->lock);
     (void)hipSetDevice(b->ctx->device);
-    (void)hipDeviceSynchronize();
+    // the last run's work (on whatever stream it was enqueued) must be done with the plan
+    if (b->done && b->ran) (void)hipEventSynchronize(b->done);
     batch_release(b);
+    if (b->done) (void)hipEventDestroy(b->done);
     delete b;
     return CMD_OK;
 }

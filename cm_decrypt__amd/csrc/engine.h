@@ -123,6 +123,8 @@ struct cmd_batch {
     const void *graph_ct = nullptr;
     void *graph_e = nullptr;
     hipStream_t graph_stream = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each run on its stream; destroy waits for it
+    bool ran = false;
 };
 
 namespace cmd {

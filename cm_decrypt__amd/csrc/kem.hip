@@ -332,11 +332,21 @@ __global__ __launch_bounds__(256) void k_keccak(KemHashArgs a)
         for (int i = 0; i < kRate / 8; i++) A[i] ^= m[blk * (kRate / 8) + i];
         keccak_f1600(A);
     }
-    uint8_t *out = a.key_out + (size_t)a.items[q] * a.key_stride;
+    // A ciphertext of 6960119 (m t = 1547 bits in 194 bytes) whose five padding bits are not
+    // zero is refused outright: its session key is all zero bytes, as the Classic McEliece
+    // specification's decapsulation for that set prescribes (the reference has no
+    // decapsulation).  The padding bits are public, so the choice leaks nothing.
+    const int item = a.items[q];
+    const int pidx = a.keys[a.item_key[q]].pidx;
+    constexpr int kPadShift = (P<3>::m * P<3>::t) % 8;
+    uint32_t last = 0;
+    if (pidx == 3) last = a.ct[(size_t)item * a.ct_stride + P<3>::synd_bytes - 1];
+    const uint64_t keep = (last >> kPadShift) != 0 ? 0ull : ~0ull;
+    uint8_t *out = a.key_out + (size_t)item * a.key_stride;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int b = 0; b < 8; b++) out[8 * i + b] = (uint8_t)(A[i] >> (8 * b));
+        for (int b = 0; b < 8; b++) out[8 * i + b] = (uint8_t)((A[i] & keep) >> (8 * b));
 }
 
 }  // namespace

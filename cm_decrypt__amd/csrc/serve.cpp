@@ -1,24 +1,30 @@
-// Continuous-batching decryption server over the fused kernel.
+// Continuous-batching server over the fused kernel: decryptions and decapsulations.
 //
-// Requests (key id + ciphertext) queue on the host.  A worker thread keeps up to
-// max_inflight launches on the device, each on its own stream with its own pinned staging
-// and device buffers: whenever a launch slot is free and work is pending, the pending
-// requests (any mix of parameter sets and resident keys) are planned into fused-kernel
-// tiles and launched, so requests that arrive while others run on the GPU join the next
-// launch and overlap with the running ones.  Finished launches are retired by polling
-// their completion events; each request's error vector is copied out and its latency
-// (submit -> result available) recorded.
+// Requests (key id + ciphertext, and whether the caller wants the error vector or the KEM
+// session key) queue on the host.  A worker thread keeps up to max_inflight launches on the
+// device, each on its own stream with its own pinned staging and device buffers: whenever a
+// launch slot is free and work is pending, the pending requests (any mix of parameter sets,
+// resident keys and request kinds) are planned into fused-kernel tiles and launched, so
+// requests that arrive while others run on the GPU join the next launch and overlap with the
+// running ones.  A launch's decapsulations sit in tiles of their own after its decryptions;
+// the re-encryption check and SHAKE256 run over those tiles after the fused kernel.
+// Finished launches are retired by polling their completion events; each request's result
+// is copied out and its latency (submit -> result available) recorded, split into the wait
+// for a launch and the launch itself.
 //
 // Host path: requests live in fixed blocks of a slot pool (never moved, reused through a
 // free list); a request id carries its slot index and a generation number, so lookups need
 // no map and a stale id is rejected.  The lock guards only the queue and the states: the
 // worker copies ciphertexts into a launch and results out of it without holding it.
 //
-// Admission: keys are admitted by the context's HBM budget when loaded; a key with
-// requests in flight cannot be unloaded; the in-flight I/O buffers are allocated up front
-// within io_budget_bytes, and submit() rejects with CMD_ERR_FULL once queue_capacity
-// requests are pending.  Every request's output lands in its own slot of its launch, so a
-// result never depends on what else shares the launch.
+// Admission: keys are admitted by the context's HBM budget when loaded.  Submission checks
+// a request's key and holds it against cmd_key_unload in one section under the context
+// lock, so a key cannot be unloaded between the check and the launch; the hold ends when the
+// request retires.  The in-flight I/O buffers are allocated up front within
+// io_budget_bytes, and submit() rejects with CMD_ERR_FULL once queue_capacity requests are
+// pending.  Every request's output lands in its own row of its launch, so a result never
+// depends on what else shares the launch.  A launch that fails (a HIP error, or the device
+// scratch of its decapsulations not fitting the budget) fails its own requests only.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -45,9 +51,12 @@ enum ReqState : uint8_t { kFree = 0, kPending = 1, kInFlight = 2, kDone = 3 };
 struct Req {
     uint32_t gen = 0;
     uint8_t state = kFree;
+    uint8_t mode = CMD_MODE_FUSED;
+    uint16_t out_bytes = 0;  // result size: n/8 (decryption) or 32 (decapsulation)
     int32_t key = -1;
     int32_t pidx = -1;
-    double t_submit = 0;
+    int32_t rc = CMD_OK;     // CMD_OK, or the error of the launch that carried the request
+    double t_submit = 0, t_launch = 0;
     uint8_t ct[kMaxSyndBytes];
     uint8_t e[kMaxEBytes];
 };
@@ -60,9 +69,12 @@ struct Slot {
     hipEvent_t done = nullptr;
     uint8_t *h_in = nullptr, *h_e = nullptr;  // pinned staging: [ciphertexts | work list], results
     DevBuf dev;                               // the same input image, then d_e
+    DevBuf kem;                               // decapsulation scratch (accept flags, SHAKE256 inputs)
     std::vector<uint32_t> reqs;               // pool indices of the launch, in item order
+    size_t n_launch = 0;                      // leading reqs that are in the launch (the rest failed)
     bool busy = false;                        // reserved by the worker (being launched or in flight)
     bool launched = false;                    // its completion event is recorded
+    double t_launch = 0;
 };
 
 }  // namespace
@@ -80,10 +92,9 @@ struct cmd_server {
     std::vector<Slot> slots;
     std::thread worker;
     bool stop = false;
-    int worker_rc = CMD_OK;
     // statistics
-    uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_items = 0;
-    std::vector<double> lat_us;
+    uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_items = 0, failed = 0;
+    std::vector<double> lat_us, queue_us, dev_us;
     double busy_since = -1, busy_us = 0;
     int busy_slots = 0;
     size_t in_bytes = 0;  // capacity of a launch's input image
@@ -141,22 +152,41 @@ constexpr size_t kEStride = kMaxEBytes;
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // Plans and enqueues the launch of slot s (its reqs already chosen).  Runs without srv->mu:
-// in-flight requests are not touched by other threads.
+// in-flight requests are not touched by other threads.  Requests whose key is no longer the
+// one they were submitted for (which the hold taken at submission rules out) are moved to
+// the end of s.reqs with CMD_ERR_KEY and left out of the launch.
 int launch_slot(cmd_server *srv, Slot &s)
 {
     cmd_ctx *ctx = srv->ctx;
-    const size_t count = s.reqs.size();
-    std::vector<int32_t> key_ids(count);
-    for (size_t i = 0; i < count; i++) key_ids[i] = srv->req(s.reqs[i]).key;
-    std::vector<FusedTile> tiles;
-    std::vector<int32_t> items, item_key;
-    int launch_tile = 32;
+    std::vector<uint32_t> dec, kem, bad;
+    dec.reserve(s.reqs.size());
+    std::vector<FusedTile> tiles, tiles_k;
+    std::vector<int32_t> items, item_key, items_k, item_key_k;
+    int launch_tile = 32, launch_tile_k = 0;
     FusedSchedule sched;
     {
         std::lock_guard<std::recursive_mutex> g(ctx->lock);
+        for (uint32_t idx : s.reqs) {
+            Req &r = srv->req(idx);
+            const bool key_ok = r.key >= 0 && (size_t)r.key < ctx->keys.size() && ctx->keys[r.key].used &&
+                                ctx->keys[r.key].pidx == r.pidx;
+            if (!key_ok) {
+                r.rc = set_error(CMD_ERR_KEY, "key id %d no longer holds the request's key", (int)r.key);
+                bad.push_back(idx);
+            } else {
+                (r.mode == CMD_MODE_KEM ? kem : dec).push_back(idx);
+            }
+        }
+        s.reqs.clear();
+        s.reqs.insert(s.reqs.end(), dec.begin(), dec.end());
+        s.reqs.insert(s.reqs.end(), kem.begin(), kem.end());
+        s.n_launch = s.reqs.size();
+        s.reqs.insert(s.reqs.end(), bad.begin(), bad.end());
+        if (s.n_launch == 0) return CMD_OK;
         // Latency first: narrow the tiles until the launch has about one wavefront per SIMD
         // (a tile's Berlekamp-Massey takes the same time for 1 or 32 ciphertexts, while its
         // syndrome and root search grow with the count), never wider than the tuned width.
+        const size_t count = s.n_launch;
         FusedSchedule per_set[kNumParams];
         const int target = ctx->n_cu * 4;
         for (int i = 0; i < kNumParams; i++) {
@@ -165,9 +195,33 @@ int launch_slot(cmd_server *srv, Slot &s)
             while (tile > 1 && (int)((count + tile - 1) / tile) < target) tile >>= 1;
             per_set[i].tile_cts = tile;
         }
-        plan_fused(ctx, per_set, count, key_ids.data(), tiles, items, item_key, &launch_tile);
+        // decryptions first, then decapsulations: tiles of one kind each, so the check and the
+        // session keys run over a contiguous range of tiles and items
+        std::vector<int32_t> keys_d(dec.size()), keys_k(kem.size());
+        for (size_t i = 0; i < dec.size(); i++) keys_d[i] = srv->req(dec[i]).key;
+        for (size_t i = 0; i < kem.size(); i++) keys_k[i] = srv->req(kem[i]).key;
+        int rc = plan_fused(ctx, per_set, dec.size(), keys_d.data(), tiles, items, item_key, &launch_tile);
+        if (!rc && !kem.empty())
+            rc = plan_fused(ctx, per_set, kem.size(), keys_k.data(), tiles_k, items_k, item_key_k, &launch_tile_k);
+        if (rc) return rc;
+        if (dec.empty()) launch_tile = launch_tile_k;
+        launch_tile = std::max(launch_tile, launch_tile_k);
+        const int32_t item_base = (int32_t)items.size();
+        for (FusedTile t : tiles_k) {
+            t.first += item_base;
+            tiles.push_back(t);
+        }
+        for (size_t i = 0; i < items_k.size(); i++) {
+            items.push_back(items_k[i] + item_base);
+            item_key.push_back(item_key_k[i]);
+        }
         sched = launch_schedule(per_set, tiles, launch_tile);
+        if (!kem.empty()) {
+            rc = dev_reserve(ctx, s.kem, align256(count * 4) + kem.size() * (size_t)kKemMsgStride);
+            if (rc) return rc;
+        }
     }
+    const size_t count = s.n_launch;
     // one input image per launch, copied in one transfer: [ct rows | items | item keys |
     // tiles | work-queue counter (zero)]
     const size_t off_items = align256(count * kCtStride);
@@ -199,22 +253,61 @@ int launch_slot(cmd_server *srv, Slot &s)
     a.tile_counter = (uint32_t *)(d_in + off_ctr);
     a.fft = ctx->d_fft;
     CMD_HIP(launch_fused(a, sched, ctx->n_cu, s.stream));
+    if (!kem.empty()) {
+        // the check reads each decapsulation's decoded e from its row; the session key then
+        // replaces the row's first 32 bytes (k_kem_pack has copied e out before k_keccak runs)
+        const int32_t n_tiles_d = (int32_t)(tiles.size() - tiles_k.size());
+        const int32_t n_items_d = (int32_t)dec.size();
+        uint32_t *d_ok = (uint32_t *)s.kem.ptr;
+        uint8_t *d_msg = (uint8_t *)s.kem.ptr + align256(count * 4);
+        DecapArgs d;
+        d.tiles = a.tiles + n_tiles_d;
+        d.n_tiles = (int32_t)tiles_k.size();
+        d.pad0 = 0;
+        d.items = a.items;
+        d.item_key = a.item_key;
+        d.keys = ctx->d_keys;
+        d.ct = a.ct;
+        d.e = d_e;
+        d.ct_stride = kCtStride;
+        d.e_stride = kEStride;
+        d.ok = d_ok;
+        d.s_out = nullptr;
+        d.fft = ctx->d_fft;
+        CMD_HIP(launch_decap_check(d, s.stream));
+        KemHashArgs h;
+        h.count = (int32_t)kem.size();
+        h.pad0 = 0;
+        h.items = a.items + n_items_d;
+        h.item_key = a.item_key + n_items_d;
+        h.keys = ctx->d_keys;
+        h.ct = a.ct;
+        h.e = d_e;
+        h.ct_stride = kCtStride;
+        h.e_stride = kEStride;
+        h.ok = d_ok;
+        h.msg = d_msg;
+        h.key_out = d_e;
+        h.key_stride = kEStride;
+        CMD_HIP(launch_kem_hash(h, s.stream));
+    }
     CMD_HIP(hipMemcpyAsync(s.h_e, d_e, count * kEStride, hipMemcpyDeviceToHost, s.stream));
     CMD_HIP(hipEventRecord(s.done, s.stream));
     return CMD_OK;
 }
 
 // Copies a finished launch's results into its requests (without srv->mu), then marks them
-// done and records statistics (with it).
-void retire_slot(cmd_server *srv, Slot &s, std::unique_lock<std::mutex> &lk, bool ok)
+// done and records statistics (with it).  launch_rc != CMD_OK fails every request of the
+// launch; requests already marked with an error keep theirs.
+void retire_slot(cmd_server *srv, Slot &s, std::unique_lock<std::mutex> &lk, int launch_rc)
 {
     lk.unlock();
     const double t = now_us();
-    if (ok)
-        for (size_t i = 0; i < s.reqs.size(); i++) {
-            Req &r = srv->req(s.reqs[i]);
-            std::memcpy(r.e, s.h_e + i * kEStride, kParams[r.pidx].n / 8);
-        }
+    for (size_t i = 0; i < s.reqs.size(); i++) {
+        Req &r = srv->req(s.reqs[i]);
+        if (r.rc == CMD_OK && launch_rc != CMD_OK) r.rc = launch_rc;
+        if (r.rc == CMD_OK && i < s.n_launch) std::memcpy(r.e, s.h_e + i * kEStride, r.out_bytes);
+    }
     {
         std::lock_guard<std::recursive_mutex> g(srv->ctx->lock);
         for (uint32_t idx : s.reqs) srv->ctx->keys[srv->req(idx).key].inflight--;
@@ -224,28 +317,19 @@ void retire_slot(cmd_server *srv, Slot &s, std::unique_lock<std::mutex> &lk, boo
         Req &r = srv->req(idx);
         r.state = kDone;
         srv->lat_us.push_back(t - r.t_submit);
+        srv->queue_us.push_back(r.t_launch - r.t_submit);
         srv->completed++;
+        if (r.rc != CMD_OK) srv->failed++;
     }
+    if (launch_rc == CMD_OK && s.n_launch) srv->dev_us.push_back(t - s.t_launch);
     s.reqs.clear();
+    s.n_launch = 0;
     s.busy = false;
     s.launched = false;
     if (--srv->busy_slots == 0) {
         srv->busy_us += t - srv->busy_since;
         srv->busy_since = -1;
     }
-}
-
-// Marks queued requests finished without a result (after a device error); waiters then get
-// worker_rc.  mu held.
-void fail_pending(cmd_server *srv)
-{
-    std::lock_guard<std::recursive_mutex> g(srv->ctx->lock);
-    for (uint32_t idx : srv->pending) {
-        Req &r = srv->req(idx);
-        srv->ctx->keys[r.key].inflight--;
-        r.state = kDone;
-    }
-    srv->pending.clear();
 }
 
 void worker_main(cmd_server *srv)
@@ -258,17 +342,20 @@ void worker_main(cmd_server *srv)
             if (!s.busy || !s.launched) continue;
             hipError_t q = hipEventQuery(s.done);
             if (q == hipSuccess) {
-                retire_slot(srv, s, lk, true);
+                retire_slot(srv, s, lk, CMD_OK);
                 progressed = true;
             } else if (q != hipErrorNotReady) {
-                srv->worker_rc = hip_fail(q, "server launch");
-                retire_slot(srv, s, lk, false);  // results are invalid; waiters see the error code
+                const int rc = hip_fail(q, "server launch");
+                lk.unlock();
+                (void)hipStreamSynchronize(s.stream);  // nothing of the launch may still touch its buffers
+                lk.lock();
+                retire_slot(srv, s, lk, rc);
                 progressed = true;
             }
         }
         if (progressed) srv->cv_done.notify_all();
 
-        if (!srv->pending.empty() && srv->worker_rc == CMD_OK) {
+        if (!srv->pending.empty()) {
             Slot *free_slot = nullptr;
             for (Slot &s : srv->slots)
                 if (!s.busy) { free_slot = &s; break; }
@@ -277,33 +364,32 @@ void worker_main(cmd_server *srv)
                                srv->busy_slots == 0 || srv->stop;
             if (free_slot && ready) {
                 Slot &s = *free_slot;
+                const double t = now_us();
                 while (!srv->pending.empty() && (int)s.reqs.size() < srv->cfg.max_batch) {
                     const uint32_t idx = srv->pending.front();
                     srv->pending.pop_front();
-                    srv->req(idx).state = kInFlight;
+                    Req &r = srv->req(idx);
+                    r.state = kInFlight;
+                    r.t_launch = t;
                     s.reqs.push_back(idx);
                 }
                 s.busy = true;
-                const double t = now_us();
+                s.t_launch = t;
                 if (srv->busy_slots++ == 0) srv->busy_since = t;
                 srv->launches++;
                 srv->launched_items += s.reqs.size();
                 lk.unlock();
-                const int rc = launch_slot(srv, s);
+                int rc = launch_slot(srv, s);
+                if (rc) (void)hipStreamSynchronize(s.stream);  // copies already queued finish first
                 lk.lock();
-                if (rc) {
-                    srv->worker_rc = rc;
-                    retire_slot(srv, s, lk, false);
+                if (rc || s.n_launch == 0) {
+                    retire_slot(srv, s, lk, rc);
                     srv->cv_done.notify_all();
                 } else {
                     s.launched = true;
                 }
                 progressed = true;
             }
-        }
-        if (srv->worker_rc != CMD_OK && !srv->pending.empty()) {
-            fail_pending(srv);
-            srv->cv_done.notify_all();
         }
         if (srv->stop && srv->pending.empty() && srv->busy_slots == 0) break;
         if (!progressed) {
@@ -314,6 +400,111 @@ void worker_main(cmd_server *srv)
                 srv->cv_work.wait(lk);
         }
     }
+}
+
+int submit_impl(cmd_server *s, int32_t mode, size_t count, const int32_t *key_ids, const uint8_t *ct,
+                size_t ct_stride, uint64_t *reqs, size_t *accepted)
+{
+    if (accepted) *accepted = 0;
+    if (!s || (count && (!key_ids || !ct || !reqs))) return CMD_ERR_ARG;
+    if (mode != CMD_MODE_FUSED && mode != CMD_MODE_KEM)
+        return set_error(CMD_ERR_ARG, "server requests are CMD_MODE_FUSED or CMD_MODE_KEM, not %d", (int)mode);
+    if (!count) return CMD_OK;
+    std::vector<int32_t> pidx(count);
+    {
+        // check every key and hold it in one section: cmd_key_unload cannot slip in between
+        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+        for (size_t i = 0; i < count; i++) {
+            const int32_t k = key_ids[i];
+            if (k < 0 || (size_t)k >= s->ctx->keys.size() || !s->ctx->keys[k].used)
+                return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)k);
+            pidx[i] = s->ctx->keys[k].pidx;
+            if (ct_stride < (size_t)kParams[pidx[i]].synd_bytes)
+                return set_error(CMD_ERR_ARG, "ct_stride %zu below the ciphertext size of item %zu", ct_stride, i);
+        }
+        for (size_t i = 0; i < count; i++) s->ctx->keys[key_ids[i]].inflight++;
+    }
+    // Grow the pool ahead of need without holding mu: a block is 4096 requests (5 MB), and
+    // allocating it under the lock would stall the worker's launches and retirements.
+    for (;;) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->free_idx.size() >= count + kBlockSize / 2 || s->pool_size >= s->pool_cap) break;
+        }
+        std::unique_ptr<Req[]> blk = cmd_server::new_block();
+        if (!blk) break;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->pool_size < s->pool_cap) s->install(std::move(blk));
+    }
+    size_t done = 0;
+    int rc = CMD_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) {
+            rc = set_error(CMD_ERR_ARG, "server is shutting down");
+        } else {
+            const double t = now_us();
+            for (; done < count; done++) {
+                if ((int)s->pending.size() >= s->cfg.queue_capacity) {
+                    s->rejected += count - done;
+                    rc = set_error(CMD_ERR_FULL, "%d requests pending", (int)s->pending.size());
+                    break;
+                }
+                const uint32_t idx = s->alloc();
+                if (idx == UINT32_MAX) {
+                    rc = set_error(CMD_ERR_NOMEM, "request pool exhausted");
+                    break;
+                }
+                Req &r = s->req(idx);
+                r.gen++;
+                r.state = kPending;
+                r.mode = (uint8_t)mode;
+                r.key = key_ids[done];
+                r.pidx = pidx[done];
+                r.out_bytes = (uint16_t)(mode == CMD_MODE_KEM ? CMD_KEM_KEY_BYTES : kParams[r.pidx].n / 8);
+                r.rc = CMD_OK;
+                r.t_submit = t;
+                r.t_launch = t;
+                std::memcpy(r.ct, ct + done * ct_stride, kParams[r.pidx].synd_bytes);
+                s->pending.push_back(idx);
+                s->submitted++;
+                reqs[done] = ((uint64_t)r.gen << 32) | idx;
+            }
+        }
+    }
+    if (done < count) {
+        // release the holds of the requests that were not queued
+        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+        for (size_t i = done; i < count; i++) s->ctx->keys[key_ids[i]].inflight--;
+    }
+    if (accepted) *accepted = done;
+    if (done) s->cv_work.notify_one();
+    return rc;
+}
+
+// Hands out a finished request's result and frees it.  mu held.
+int take_result(cmd_server *s, uint64_t id, uint8_t *out, size_t cap)
+{
+    Req *r = s->lookup(id);
+    if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+    if (r->state != kDone) return CMD_ERR_PENDING;
+    const int rc = r->rc;
+    if (rc == CMD_OK && out) {
+        if (cap < r->out_bytes)
+            return set_error(CMD_ERR_ARG, "result buffer of %zu bytes, %d needed", cap, (int)r->out_bytes);
+        std::memcpy(out, r->e, r->out_bytes);
+    }
+    r->state = kFree;
+    s->free_idx.push_back((uint32_t)id);
+    return rc;
+}
+
+double percentile(std::vector<double> v, double q)
+{
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const size_t i = (size_t)(q * (double)(v.size() - 1) + 0.5);
+    return v[std::min(i, v.size() - 1)];
 }
 
 }  // namespace
@@ -348,7 +539,11 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     srv->pool_cap = (uint32_t)(nblocks << kBlockBits);
     srv->slots.resize(c.max_inflight);
     for (Slot &s : srv->slots) s.reqs.reserve(mb);
-    srv->lat_us.reserve(1 << 18);
+    // sample vectors sized for long measurement windows: growing them under mu would stall
+    // the worker
+    srv->lat_us.reserve(1 << 20);
+    srv->queue_us.reserve(1 << 20);
+    srv->dev_us.reserve(1 << 16);
     // a first few blocks up front; submit grows the pool ahead of need
     for (int b = 0; b < 4 && srv->pool_size < srv->pool_cap; b++) {
         std::unique_ptr<Req[]> blk = cmd_server::new_block();
@@ -385,171 +580,130 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     return CMD_OK;
 }
 
+extern "C" int cmd_server_submit_batch_mode(cmd_server *s, int32_t mode, size_t count, const int32_t *key_ids,
+                                            const uint8_t *ct, size_t ct_stride, uint64_t *reqs, size_t *accepted)
+{
+    return submit_impl(s, mode, count, key_ids, ct, ct_stride, reqs, accepted);
+}
+
 extern "C" int cmd_server_submit_batch(cmd_server *s, size_t count, const int32_t *key_ids, const uint8_t *ct,
                                        size_t ct_stride, uint64_t *reqs, size_t *accepted)
 {
-    if (accepted) *accepted = 0;
-    if (!s || (count && (!key_ids || !ct || !reqs))) return CMD_ERR_ARG;
-    std::vector<int32_t> pidx(count);
-    {
-        std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-        for (size_t i = 0; i < count; i++) {
-            const int32_t k = key_ids[i];
-            if (k < 0 || (size_t)k >= s->ctx->keys.size() || !s->ctx->keys[k].used)
-                return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)k);
-            pidx[i] = s->ctx->keys[k].pidx;
-            if (ct_stride < (size_t)kParams[pidx[i]].synd_bytes)
-                return set_error(CMD_ERR_ARG, "ct_stride %zu below the ciphertext size of item %zu", ct_stride, i);
-        }
-    }
-    // Grow the pool ahead of need without holding mu: a block is 4096 requests (5 MB), and
-    // allocating it under the lock would stall the worker's launches and retirements.
-    for (;;) {
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            if (s->free_idx.size() >= count + kBlockSize / 2 || s->pool_size >= s->pool_cap) break;
-        }
-        std::unique_ptr<Req[]> blk = cmd_server::new_block();
-        if (!blk) break;
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->pool_size < s->pool_cap) s->install(std::move(blk));
-    }
-    size_t done = 0;
-    int rc = CMD_OK;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop) return set_error(CMD_ERR_ARG, "server is shutting down");
-        const double t = now_us();
-        for (; done < count; done++) {
-            if ((int)s->pending.size() >= s->cfg.queue_capacity) {
-                s->rejected += count - done;
-                rc = set_error(CMD_ERR_FULL, "%d requests pending", (int)s->pending.size());
-                break;
-            }
-            const uint32_t idx = s->alloc();
-            if (idx == UINT32_MAX) {
-                rc = set_error(CMD_ERR_NOMEM, "request pool exhausted");
-                break;
-            }
-            Req &r = s->req(idx);
-            r.gen++;
-            r.state = kPending;
-            r.key = key_ids[done];
-            r.pidx = pidx[done];
-            r.t_submit = t;
-            std::memcpy(r.ct, ct + done * ct_stride, kParams[r.pidx].synd_bytes);
-            s->pending.push_back(idx);
-            s->submitted++;
-            reqs[done] = ((uint64_t)r.gen << 32) | idx;
-        }
-        if (done) {
-            // held against cmd_key_unload until retired (context lock after the server lock,
-            // the order every path here uses)
-            std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-            for (size_t i = 0; i < done; i++) s->ctx->keys[key_ids[i]].inflight++;
-        }
-    }
-    if (accepted) *accepted = done;
-    if (done) s->cv_work.notify_one();
-    return rc;
+    return submit_impl(s, CMD_MODE_FUSED, count, key_ids, ct, ct_stride, reqs, accepted);
 }
 
 extern "C" int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *ct, uint64_t *req)
 {
     if (!s || !ct || !req) return CMD_ERR_ARG;
-    return cmd_server_submit_batch(s, 1, &key_id, ct, kMaxSyndBytes, req, nullptr);
+    return submit_impl(s, CMD_MODE_FUSED, 1, &key_id, ct, kMaxSyndBytes, req, nullptr);
 }
 
-static int take_result(cmd_server *s, uint64_t id, uint8_t *e_out)
+extern "C" int cmd_server_result_bytes(cmd_server *s, uint64_t req, size_t *bytes)
 {
-    Req *r = s->lookup(id);
-    if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)id);
-    if (r->state != kDone) return CMD_ERR_PENDING;
-    if (s->worker_rc) return s->worker_rc;
-    if (e_out) std::memcpy(e_out, r->e, kParams[r->pidx].n / 8);
-    r->state = kFree;
-    s->free_idx.push_back((uint32_t)id);
+    if (!s || !bytes) return CMD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const Req *r = s->lookup(req);
+    if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)req);
+    *bytes = r->out_bytes;
     return CMD_OK;
 }
 
-extern "C" int cmd_server_poll(cmd_server *s, uint64_t req, uint8_t *e_out)
+extern "C" int cmd_server_poll(cmd_server *s, uint64_t req, uint8_t *out, size_t out_cap)
 {
     if (!s) return CMD_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    return take_result(s, req, e_out);
+    return take_result(s, req, out, out_cap);
 }
 
-extern "C" int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *e_out, double timeout_us)
+extern "C" int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *out, size_t out_cap, double timeout_us)
 {
     if (!s) return CMD_ERR_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
     const auto deadline = Clock::now() + std::chrono::microseconds((long long)(timeout_us < 0 ? 0 : timeout_us));
     for (;;) {
-        int rc = take_result(s, req, e_out);
+        int rc = take_result(s, req, out, out_cap);
         if (rc != CMD_ERR_PENDING) return rc;
         if (timeout_us < 0) {
             s->cv_done.wait(lk);
         } else if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
-            rc = take_result(s, req, e_out);
+            rc = take_result(s, req, out, out_cap);
             return rc == CMD_ERR_PENDING ? CMD_ERR_TIMEOUT : rc;
         }
     }
 }
 
-extern "C" int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t *reqs, uint8_t *e_out,
-                                     size_t e_stride, double timeout_us)
+extern "C" int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t *reqs, uint8_t *out,
+                                     size_t out_stride, double timeout_us)
 {
-    if (!s || (count && (!reqs || !e_out))) return CMD_ERR_ARG;
+    if (!s || (count && (!reqs || !out))) return CMD_ERR_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
     const auto deadline = Clock::now() + std::chrono::microseconds((long long)(timeout_us < 0 ? 0 : timeout_us));
     for (size_t i = 0; i < count; i++) {
         const Req *r = s->lookup(reqs[i]);
         if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)reqs[i]);
-        if ((size_t)kParams[r->pidx].n / 8 > e_stride)
-            return set_error(CMD_ERR_ARG, "e_stride %zu below the error vector of request %zu", e_stride, i);
+        if ((size_t)r->out_bytes > out_stride)
+            return set_error(CMD_ERR_ARG, "out_stride %zu below the result of request %zu", out_stride, i);
     }
+    int first_err = CMD_OK;
     for (size_t i = 0; i < count; i++) {
         for (;;) {
-            int rc = take_result(s, reqs[i], e_out + i * e_stride);
-            if (rc != CMD_ERR_PENDING) {
-                if (rc) return rc;
-                break;
-            }
-            if (timeout_us < 0) {
-                s->cv_done.wait(lk);
-            } else if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
-                rc = take_result(s, reqs[i], e_out + i * e_stride);
+            int rc = take_result(s, reqs[i], out + i * out_stride, out_stride);
+            if (rc == CMD_ERR_PENDING) {
+                if (timeout_us < 0) {
+                    s->cv_done.wait(lk);
+                    continue;
+                }
+                if (s->cv_done.wait_until(lk, deadline) != std::cv_status::timeout) continue;
+                rc = take_result(s, reqs[i], out + i * out_stride, out_stride);
                 if (rc == CMD_ERR_PENDING) return CMD_ERR_TIMEOUT;
-                if (rc) return rc;
-                break;
             }
+            if (rc != CMD_OK && first_err == CMD_OK) first_err = rc;
+            break;
         }
     }
-    return CMD_OK;
+    return first_err;
 }
 
 extern "C" int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out)
 {
     if (!s || !out) return CMD_ERR_ARG;
+    std::vector<double> lat, queue, dev;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        out->submitted = s->submitted;
+        out->completed = s->completed;
+        out->rejected = s->rejected;
+        out->launches = s->launches;
+        out->failed = s->failed;
+        out->mean_batch = s->launches ? (double)s->launched_items / (double)s->launches : 0.0;
+        double busy = s->busy_us;
+        if (s->busy_since >= 0) busy += now_us() - s->busy_since;
+        out->busy_s = busy * 1e-6;
+        lat = s->lat_us;
+        queue = s->queue_us;
+        dev = s->dev_us;
+    }
+    out->p50_us = percentile(lat, 0.50);
+    out->p99_us = percentile(lat, 0.99);
+    out->p999_us = percentile(lat, 0.999);
+    out->max_us = percentile(lat, 1.0);
+    out->queue_p99_us = percentile(queue, 0.99);
+    out->queue_max_us = percentile(queue, 1.0);
+    out->device_p99_us = percentile(dev, 0.99);
+    out->device_max_us = percentile(dev, 1.0);
+    return CMD_OK;
+}
+
+extern "C" int cmd_server_latencies(cmd_server *s, double *total_us, double *queue_us, size_t cap, size_t *n)
+{
+    if (!s || (cap && (!total_us || !queue_us))) return CMD_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    out->submitted = s->submitted;
-    out->completed = s->completed;
-    out->rejected = s->rejected;
-    out->launches = s->launches;
-    out->mean_batch = s->launches ? (double)s->launched_items / (double)s->launches : 0.0;
-    std::vector<double> v = s->lat_us;
-    std::sort(v.begin(), v.end());
-    auto pct = [&](double q) {
-        if (v.empty()) return 0.0;
-        size_t i = (size_t)(q * (double)(v.size() - 1) + 0.5);
-        return v[std::min(i, v.size() - 1)];
-    };
-    out->p50_us = pct(0.50);
-    out->p99_us = pct(0.99);
-    out->max_us = v.empty() ? 0.0 : v.back();
-    double busy = s->busy_us;
-    if (s->busy_since >= 0) busy += now_us() - s->busy_since;
-    out->busy_s = busy * 1e-6;
+    const size_t k = std::min(cap, s->lat_us.size());
+    if (k) {
+        std::memcpy(total_us, s->lat_us.data(), k * sizeof(double));
+        std::memcpy(queue_us, s->queue_us.data(), k * sizeof(double));
+    }
+    if (n) *n = s->lat_us.size();
     return CMD_OK;
 }
 
@@ -557,8 +711,10 @@ extern "C" int cmd_server_reset_stats(cmd_server *s)
 {
     if (!s) return CMD_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    s->submitted = s->completed = s->rejected = s->launches = s->launched_items = 0;
+    s->submitted = s->completed = s->rejected = s->launches = s->launched_items = s->failed = 0;
     s->lat_us.clear();
+    s->queue_us.clear();
+    s->dev_us.clear();
     s->busy_us = 0;
     if (s->busy_since >= 0) s->busy_since = now_us();
     return CMD_OK;
@@ -582,6 +738,7 @@ extern "C" int cmd_server_destroy(cmd_server *s)
         if (sl.h_in) (void)hipHostFree(sl.h_in);
         if (sl.h_e) (void)hipHostFree(sl.h_e);
         dev_free(s->ctx, sl.dev);
+        dev_free(s->ctx, sl.kem);
     }
     delete s;  // every request was retired by the worker; the pool blocks go with the server
     return CMD_OK;

@@ -130,6 +130,10 @@ int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx **out);
 int cmd_ctx_destroy(cmd_ctx *ctx);
 /* Device memory charged against the budget (keys + work buffers) and the budget. */
 int cmd_ctx_memory(cmd_ctx *ctx, uint64_t *used, uint64_t *budget);
+/* Changes the HBM budget (admission control) of a live context; nothing already allocated
+ * is freed, later allocations that would exceed it fail with CMD_ERR_BUDGET.  No counterpart
+ * in the reference, which allocates fixed global arrays. */
+int cmd_ctx_set_budget(cmd_ctx *ctx, uint64_t hbm_budget_bytes);
 
 /* Uploads one key's Goppa coefficients and control bits and expands them on the GPU
  * (support, syndrome table, bit planes, Benes masks); the key stays resident until unloaded.
@@ -261,28 +265,48 @@ typedef struct {
 
 typedef struct {
     uint64_t submitted, completed, rejected, launches;
+    uint64_t failed;                   /* requests finished with an error (their launch failed) */
     double   mean_batch;
-    double   p50_us, p99_us, max_us;   /* submit -> result available */
+    double   p50_us, p99_us, p999_us, max_us;  /* submit -> result available */
+    double   queue_p99_us, queue_max_us;       /* submit -> launch enqueued */
+    double   device_p99_us, device_max_us;     /* launch enqueued -> completion seen (per launch) */
     double   busy_s;                   /* time with at least one launch in flight */
 } cmd_server_stats;
 
 int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd_server **out);
-/* Copies the ciphertext (ct_bytes of the key's set); *req receives the request id. */
+/* Copies the ciphertext (ct_bytes of the key's set); *req receives the request id.  The
+ * request decrypts (CMD_MODE_FUSED); its result is the n/8-byte error vector. */
 int cmd_server_submit(cmd_server *s, int32_t key_id, const uint8_t *ct, uint64_t *req);
-/* Submits count requests at once (item i: key_ids[i], ciphertext at ct + i*ct_stride);
- * reqs[i] receives the ids.  Keys and strides are checked before anything is queued; if the
- * queue fills, the first *accepted requests stay submitted and CMD_ERR_FULL is returned. */
+/* Submits count decryption requests at once (item i: key_ids[i], ciphertext at
+ * ct + i*ct_stride); reqs[i] receives the ids.  Keys and strides are checked, and the keys
+ * held against cmd_key_unload, before anything is queued; if the queue fills, the first
+ * *accepted requests stay submitted and CMD_ERR_FULL is returned. */
 int cmd_server_submit_batch(cmd_server *s, size_t count, const int32_t *key_ids, const uint8_t *ct,
                             size_t ct_stride, uint64_t *reqs, size_t *accepted);
-/* CMD_OK and the error vector once done, CMD_ERR_PENDING before; frees the request. */
-int cmd_server_poll(cmd_server *s, uint64_t req, uint8_t *e_out);
+/* The same for requests of `mode`: CMD_MODE_FUSED (result: the n/8-byte error vector) or
+ * CMD_MODE_KEM (result: the 32-byte session key of crypto_kem_dec, as cmd_kem_dec computes
+ * it).  Both kinds share launches: a launch's decapsulations run the re-encryption check and
+ * SHAKE256 after its fused decryption kernel. */
+int cmd_server_submit_batch_mode(cmd_server *s, int32_t mode, size_t count, const int32_t *key_ids,
+                                 const uint8_t *ct, size_t ct_stride, uint64_t *reqs, size_t *accepted);
+/* Bytes of a request's result (n/8 for a decryption, 32 for a decapsulation). */
+int cmd_server_result_bytes(cmd_server *s, uint64_t req, size_t *bytes);
+/* CMD_OK and the result (out_cap >= its size, else CMD_ERR_ARG) once done, CMD_ERR_PENDING
+ * before; frees the request.  A request whose launch failed returns that launch's error code
+ * (and is freed); the server keeps serving other launches. */
+int cmd_server_poll(cmd_server *s, uint64_t req, uint8_t *out, size_t out_cap);
 /* Blocks until done or timeout_us elapses (< 0: no limit). */
-int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *e_out, double timeout_us);
-/* Waits for count requests and writes their error vectors to e_out + i*e_stride (the
- * requests are freed); CMD_ERR_TIMEOUT once timeout_us (< 0: no limit) has passed. */
-int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t *reqs, uint8_t *e_out, size_t e_stride,
+int cmd_server_wait(cmd_server *s, uint64_t req, uint8_t *out, size_t out_cap, double timeout_us);
+/* Waits for count requests and writes their results to out + i*out_stride (the requests are
+ * freed); CMD_ERR_TIMEOUT once timeout_us (< 0: no limit) has passed.  The first failed
+ * request's code is returned after every request has been waited for. */
+int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t *reqs, uint8_t *out, size_t out_stride,
                           double timeout_us);
 int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out);
+/* Per-request latency samples of the requests retired since the last reset, in retirement
+ * order: total (submit -> result) and queue (submit -> launch) microseconds.  Copies at most
+ * cap of each; *n receives the number available. */
+int cmd_server_latencies(cmd_server *s, double *total_us, double *queue_us, size_t cap, size_t *n);
 int cmd_server_reset_stats(cmd_server *s);
 /* Stops admission, drains in-flight work, frees everything. */
 int cmd_server_destroy(cmd_server *s);

@@ -113,7 +113,10 @@ def kem_dec(pid, sk, ct):
     (session keys count x 32, accepted flags).  e from the oracle decoder is accepted when its
     weight is t and s(e) over all n positions equals the ciphertext's syndrome; the key is
     SHAKE256(1 || e || c) then (the reference encapsulation's formula,
-    kem/common/core/operations.c:15-34), else SHAKE256(0 || s || c) with the secret key's s."""
+    kem/common/core/operations.c:15-34), else SHAKE256(0 || s || c) with the secret key's s.
+    For 6960119 a ciphertext with nonzero padding bits (above bit mt of its last byte) gets an
+    all-zero key, the specification's rule for that set; the reference has no decapsulation,
+    so this branch is pinned by the specification alone."""
     m, n, t, ctb, sb, skb = PARAMS[pid]
     sk = np.ascontiguousarray(sk, np.uint8)
     ct = np.ascontiguousarray(ct, np.uint8).reshape(-1, ctb)
@@ -129,6 +132,11 @@ def kem_dec(pid, sk, ct):
         pre = (bytes([1]) + e.tobytes() if good else bytes([0]) + s_rej.tobytes()) + ct[i].tobytes()
         keys[i] = np.frombuffer(shake256_32(pre), np.uint8)
         ok[i] = good
+        if (m * t) % 8 and int(ct[i, ctb - 1]) >> ((m * t) % 8):
+            # nonzero padding bits (6960119): the specification's decapsulation returns an
+            # all-zero key for such a ciphertext
+            keys[i] = 0
+            ok[i] = False
     return keys, ok
 
 

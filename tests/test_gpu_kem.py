@@ -108,3 +108,25 @@ def test_kem_dec_rejects_short_key_stride(ctx, kem_keys):
     with pytest.raises(cm.CmdError):
         b = cm.Batch(ctx, [kid, kid], 96, 16, mode=cm.MODE_KEM)
         b.close()
+
+
+def test_kem_dec_nonzero_padding_6960119(ctx, kem_keys):
+    """mceliece6960119 ciphertexts hold 1547 bits in 194 bytes: a valid ciphertext with any of
+    its five padding bits set gets the all-zero key, while the same ciphertext with clean
+    padding keeps the reference encapsulation's key.  The rule is the specification's for this
+    set (the reference has no decapsulation)."""
+    pid = 6960119
+    sk, kid = kem_keys[pid]
+    ct, e = cm.gen_encrypt(pid, sk, 123, 6)
+    padded = ct.copy()
+    for i in range(5):
+        padded[i, -1] |= np.uint8(1 << (3 + i))  # bits 3..7 of the last byte are padding
+    cts = np.vstack([ct, padded])
+    print(f"kem padding items={len(cts)}", flush=True)
+    got = ctx.kem_dec([kid] * len(cts), cts)
+    want, ok = orc.kem_dec(pid, sk, cts)
+    assert np.array_equal(got, want)
+    assert ok[:6].all() and not ok[6:11].any() and ok[11]
+    assert not got[6:11].any() and got[:6].any(axis=1).all()
+    for i in range(6):
+        assert got[i].tobytes() == orc.shake256_32(b"\x01" + e[i].tobytes() + ct[i].tobytes())

@@ -380,9 +380,18 @@ def test_kem_dec_oracle(pid):
     flip = c0.copy()
     flip[0] ^= 0x01 if not (e0[0] & 1) else 0x02
     rnd = np.random.default_rng(pid).integers(0, 256, (2, ctb), dtype=np.uint8)
+    pad = (m * t) % 8
+    if pad:
+        rnd[:, -1] &= (1 << pad) - 1  # padding bits clear: rejected by the decoder's check
     bad = np.vstack([low, flip, rnd])
     keys, ok = orc.kem_dec(pid, sk, bad)
     assert not ok.any()
     s_rej = sk[40 + 2 * t + orc.COND_BYTES[pid]:][:n // 8].tobytes()
     for c, k in zip(bad, keys):
         assert k.tobytes() == orc.shake256_32(b"\x00" + s_rej + c.tobytes())
+    if pad:
+        # a valid ciphertext with a padding bit set: all-zero key (the specification's rule)
+        c1 = gold["ct"][0].copy()
+        c1[-1] |= np.uint8(1 << pad)
+        keys, ok = orc.kem_dec(pid, sk, c1[None, :])
+        assert not ok[0] and not keys.any()
