@@ -34,6 +34,7 @@ MODE_FUSED = 1
 MODE_KEM = 2           # rows out are 32-byte session keys (crypto_kem_dec)
 KEM_KEY_BYTES = 32
 MODE_POSITIONS = 3     # rows out are t uint16 error positions, ascending, 0xFFFF-padded
+NOALLOC_MAX_ITEMS = 262144  # items per cmd_decrypt_device_noalloc call
 
 CMD_OK = 0
 CMD_ERR_FILE = -1
@@ -140,6 +141,7 @@ _sig("cmd_key_device_bytes", _I, _I32, _c.POINTER(_U64))
 _sig("cmd_key_export", _I, _P, _I32, _P, _P, _P, _P, _c.POINTER(_I32))
 _sig("cmd_decrypt", _I, _P, _I32, _SZ, _P, _P, _SZ, _P, _SZ)
 _sig("cmd_decrypt_device", _I, _P, _I32, _SZ, _P, _P, _SZ, _P, _SZ, _P)
+_sig("cmd_decrypt_device_noalloc", _I, _P, _SZ, _P, _P, _SZ, _P, _SZ, _P)
 _sig("cmd_batch_create", _I, _P, _I32, _SZ, _P, _SZ, _SZ, _I32, _c.POINTER(_P))
 _sig("cmd_batch_run", _I, _P, _P, _P, _P)
 _sig("cmd_batch_destroy", _I, _P)

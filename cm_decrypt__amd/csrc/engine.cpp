@@ -683,6 +683,7 @@ extern "C" int cmd_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cmd_ctx
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(CMD_ERR_NODEV, "no HIP device");
     if (device < 0 || device >= ndev) return set_error(CMD_ERR_NODEV, "device %d of %d", (int)device, ndev);
     CMD_HIP(hipSetDevice(device));
+    if (int rc = noalloc_setup(device)) return rc;
     hipDeviceProp_t prop;
     CMD_HIP(hipGetDeviceProperties(&prop, device));
     std::unique_ptr<cmd_ctx> ctx(new cmd_ctx());

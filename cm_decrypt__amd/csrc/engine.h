@@ -83,6 +83,10 @@ struct cmd_ctx {
     std::vector<cmd::IoSlot> io;                            // host-buffer pipeline stages
     std::vector<std::pair<uintptr_t, size_t>> host_pinned;  // cmd_host_alloc blocks
 
+    // host plan of cmd_decrypt_device_noalloc, reused from call to call
+    std::vector<cmd::FusedTile> na_tiles;
+    std::vector<int32_t> na_items, na_item_key;
+
     bool timing = false;
     bool fit_tiles = true;  // batches narrow their tiles when too small to fill the GPU (off while autotuning)
     double timing_us = 0.0;
@@ -178,5 +182,9 @@ int enqueue_fused(cmd_ctx *ctx, int mode, const FusedSchedule sched[], const std
 void batch_release(cmd_batch *b);
 
 FusedSchedule default_schedule(int pidx);
+
+// Looks up this device's copy of the allocation-free entry point's device array (once per
+// device; the device must be current).
+int noalloc_setup(int device);
 
 }  // namespace cmd

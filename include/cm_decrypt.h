@@ -184,6 +184,19 @@ int cmd_host_free(cmd_ctx *ctx, void *ptr);
 int cmd_decrypt_device(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *key_ids,
                        const void *d_ct, size_t ct_stride, void *d_e, size_t e_stride, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * fused decryption as cmd_decrypt_device with CMD_MODE_FUSED, but the launch's work list
+ * (tiles, item order, per-item keys, work-queue counter) goes to a fixed device array of the
+ * library (about 6 MiB per GPU, held until the process ends) instead of memory allocated per
+ * call; nothing is allocated on the device.  At most CMD_NOALLOC_MAX_ITEMS items per call
+ * (CMD_ERR_ARG beyond).  All calls on one GPU share that array: calls that overlap on one
+ * GPU (several contexts or processes' threads at once) race; one context serialises its own
+ * calls, and each call returns once its kernels are done. */
+#define CMD_NOALLOC_MAX_ITEMS 262144
+int cmd_decrypt_device_noalloc(cmd_ctx *ctx, size_t count, const int32_t *key_ids, const void *d_ct,
+                               size_t ct_stride, void *d_e, size_t e_stride, void *stream);
+
 /* KEM decapsulation, host buffers (crypto_kem_dec of Classic McEliece, batched).  Item i:
  * ciphertext ct + i*ct_stride of key key_ids[i]; its 32-byte session key goes to
  * key_out + i*key_stride.  The key is SHAKE256(1 || e || c) when the decoded e has weight t
