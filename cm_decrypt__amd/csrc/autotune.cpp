@@ -26,6 +26,13 @@ static std::vector<FusedSchedule> candidates()
                     if (wpg * gpc > 8) continue;  // at most two wavefronts per SIMD
                     if (fused_supported(s)) out.push_back(s);
                 }
+    // two wavefronts per tile (one pair per workgroup, 128-register build): up to eight pairs,
+    // sixteen wavefronts, per CU; a pair's LDS is 17 KB
+    for (int tile : {8, 16, 32})
+        for (int gpc : {2, 3, 4, 5, 6, 7, 8}) {
+            FusedSchedule s{tile, 2, 2, gpc, 1};
+            if (fused_supported(s)) out.push_back(s);
+        }
     return out;
 }
 
@@ -129,30 +136,49 @@ extern "C" int cmd_schedule_save(cmd_ctx *ctx, const char *path)
     return std::fclose(f) == 0 ? CMD_OK : set_error(CMD_ERR_FILE, "cannot write %s", path);
 }
 
-// Reads the format written by cmd_schedule_save (one set per line).
+// Reads the JSON written by cmd_schedule_save (or any layout of the same object: per set id,
+// an object with the five fields in any order and spacing).
+static bool json_int_field(const std::string &obj, const char *name, int *out)
+{
+    const std::string key = std::string("\"") + name + "\"";
+    size_t at = obj.find(key);
+    if (at == std::string::npos) return false;
+    at = obj.find(':', at + key.size());
+    if (at == std::string::npos) return false;
+    char *end = nullptr;
+    const long v = std::strtol(obj.c_str() + at + 1, &end, 10);
+    if (end == obj.c_str() + at + 1) return false;
+    *out = (int)v;
+    return true;
+}
+
 extern "C" int cmd_schedule_load(cmd_ctx *ctx, const char *path)
 {
     if (!ctx || !path) return CMD_ERR_ARG;
     FILE *f = std::fopen(path, "r");
     if (!f) return set_error(CMD_ERR_FILE, "cannot read %s", path);
-    char line[512];
+    std::string text;
+    char buf[4096];
+    for (size_t n; (n = std::fread(buf, 1, sizeof buf, f)) > 0;) text.append(buf, n);
+    std::fclose(f);
     int loaded = 0;
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
-    while (std::fgets(line, sizeof line, f)) {
-        int id, tile, wpg, wpc, gpc, alg;
-        const char *q = std::strchr(line, '"');
-        if (!q) continue;
-        if (std::sscanf(q, "\"%d\": {\"tile_cts\": %d, \"waves_per_group\": %d, \"waves_per_ct\": %d, "
-                           "\"groups_per_cu\": %d, \"root_alg\": %d}",
-                        &id, &tile, &wpg, &wpc, &gpc, &alg) == 6) {
-            int pidx = param_index(id);
-            FusedSchedule s{tile, wpg, wpc, gpc, alg};
-            if (pidx >= 0 && fused_supported(s) && gpc >= 1 && gpc <= 16) {
-                ctx->sched[pidx] = s;
-                loaded++;
-            }
+    for (int pidx = 0; pidx < kNumParams; pidx++) {
+        const std::string key = "\"" + std::to_string(kParams[pidx].id) + "\"";
+        const size_t at = text.find(key);
+        if (at == std::string::npos) continue;
+        const size_t open = text.find('{', at), close = text.find('}', at);
+        if (open == std::string::npos || close == std::string::npos || close < open) continue;
+        const std::string obj = text.substr(open, close - open + 1);
+        FusedSchedule s{};
+        if (!json_int_field(obj, "tile_cts", &s.tile_cts) || !json_int_field(obj, "waves_per_group", &s.waves_per_group) ||
+            !json_int_field(obj, "waves_per_ct", &s.waves_per_ct) || !json_int_field(obj, "groups_per_cu", &s.groups_per_cu) ||
+            !json_int_field(obj, "root_alg", &s.root_alg))
+            continue;
+        if (fused_supported(s) && s.groups_per_cu >= 1 && s.groups_per_cu <= 16) {
+            ctx->sched[pidx] = s;
+            loaded++;
         }
     }
-    std::fclose(f);
     return loaded ? CMD_OK : set_error(CMD_ERR_FILE, "no schedules in %s", path);
 }

@@ -1111,11 +1111,9 @@ struct FusedTileKernel {
     static constexpr int NP = FP::NP, NW = FP::NW, WPL = FP::WPL, WPLEAF = FP::WPLEAF;
     static_assert(TILE * (1 << LOG) * 2 <= kWaveLds * 4, "leaf rows must fit the wave's LDS");
 
-    static __device__ __forceinline__ void fft_coeff_phase(const uint32_t (&C)[NBL][M], uint32_t *lds, int lane,
-                                                           const FftTab *tab)
+    // C planes to LDS rows (row i = coefficient i, word k = plane k).
+    static __device__ __forceinline__ void write_coef_planes(const uint32_t (&C)[NBL][M], uint32_t *lds, int lane)
     {
-        // C planes to LDS rows (row i = coefficient i), then gather f_p = C_{T-p}
-        wave_sync();
 #pragma unroll
         for (int b = 0; b < NBL; b++) {
             const int i = lane + 64 * b;
@@ -1123,7 +1121,21 @@ struct FusedTileKernel {
 #pragma unroll
             for (int k = 0; k < M; k++) lds[i * kRowWords + k] = C[b][k];
         }
+    }
+
+    static __device__ __forceinline__ void fft_coeff_phase(const uint32_t (&C)[NBL][M], uint32_t *lds, int lane,
+                                                           const FftTab *tab)
+    {
         wave_sync();
+        write_coef_planes(C, lds, lane);
+        wave_sync();
+        fft_coeff_from_rows(lds, lane, tab);
+    }
+
+    // The coefficient phase from the coefficient rows already in LDS: gather f_p = C_{T-p},
+    // then the twists and radix passes; leaves replace the rows.
+    static __device__ __forceinline__ void fft_coeff_from_rows(uint32_t *lds, int lane, const FftTab *tab)
+    {
         uint32_t F[NP][M];
 #pragma unroll
         for (int pb = 0; pb < NP; pb++) {
@@ -1159,10 +1171,11 @@ struct FusedTileKernel {
     }
 
     static __device__ __forceinline__ void roots_fft(const FusedArgs &a, const FusedTile &tl, const uint32_t *lds,
-                                                     int lane, bool e_aligned, const FftTab *tab)
+                                                     int lane, bool e_aligned, const FftTab *tab, int x0 = 0,
+                                                     int dx = 1)
     {
         const uint16_t *leaf = reinterpret_cast<const uint16_t *>(lds);
-        for (int x = 0; x < tl.count; x++) {
+        for (int x = x0; x < tl.count; x += dx) {
             const uint32_t wl_fin = FP::lane_part(FP::SCHED.fin, opaque(lane), 64);
             const int item = a.items[tl.first + x];
             const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
@@ -1226,7 +1239,381 @@ struct FusedTileKernel {
         }
         wave_sync();
     }
+
+    // ---- two wavefronts per tile (waves_per_ct == 2) -----------------------------------------
+    // One workgroup of two wavefronts shares a tile and its LDS, so that four wavefronts per
+    // SIMD fit where the single-wavefront kernel fits two, without adding Berlekamp-Massey work
+    // (a tile's recurrence is split, not repeated):
+    //  1. the syndrome is split by table column: wave w accumulates the dword column
+    //     lane + 64 w of every row (syndromes 2c, 2c+1 of column c) for every ciphertext of the
+    //     tile, with four-row tables of one dword per lane (4 KB per wave), and stores the bit
+    //     planes of its own syndromes (when t <= 64 every column belongs to wave 0);
+    //  2. wave 0 holds lane block 0 of C and B, wave 1 block 1 (when there is one) and, when t
+    //     is a multiple of 64, the transposed top coefficient; per iteration each wave reduces
+    //     its part of the discrepancy, the parts meet in LDS (one barrier per iteration, slots
+    //     alternating so a slot is rewritten only after both waves passed the next barrier),
+    //     and wave 0 publishes lane 63 of its C and B, from which wave 1 takes the carry of the
+    //     shift of B (and for t = 64 the new B_t);
+    //  3. wave 0 runs the FFT coefficient phase; both waves run the butterflies, Benes network
+    //     and stores of alternate ciphertexts.
+    // Tiles of at most SCALAR_BM ciphertexts run their Berlekamp-Massey in wave 0 alone.
+    static constexpr int XCH = 64;  // words per exchange slot: d of wave 0 | C_63 | B_63 | d of wave 1
+
+    // The eight rows of group g, one dword column per lane.
+    static __device__ __forceinline__ void load_group_col(gu32 *rows, int g, int col, uint32_t (&v)[8])
+    {
+        gu32 *base = rows + (size_t)(8 * g) * PAIRS;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = col < PAIRS ? base[(size_t)q * PAIRS + col] : 0u;
+    }
+
+    // Group v into the accumulators: rows 4h .. 4h+3 through one 16-entry table at a time
+    // (entry c = XOR of the rows set in c, lane-major); ciphertext x adds the entry its nibble
+    // selects.  This wave's LDS operations execute in order, so a table is rewritten only after
+    // the reads of the previous one.
+    template <int BIT0>
+    static __device__ __forceinline__ void add_group_col(const uint32_t (&v)[8], const uint32_t (&cw)[CH],
+                                                         uint32_t (&acc)[CH], uint32_t *tab, int lane)
+    {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t cur = 0;
+#pragma unroll
+            for (int i = 1; i < 16; i++) {
+                cur ^= v[4 * h + __builtin_ctz(i)];
+                tab[(i ^ (i >> 1)) * 64 + lane] = cur;
+            }
+#pragma unroll
+            for (int x = 0; x < CH; x++) {
+                acc[x] ^= tab[((cw[x] >> (BIT0 + 4 * h)) & 15u) * 64 + lane];
+                asm volatile("" : "+v"(acc[x]));
+            }
+        }
+    }
+
+    // All row groups of one key's table for the ciphertexts of a run (others read as zero),
+    // software-pipelined: the next group's loads are in flight while the current one is added.
+    static __device__ __forceinline__ void stream_col(gu32 *rows, const uint8_t *my_ct, int col,
+                                                      uint32_t (&acc)[CH], uint32_t *tab, int lane)
+    {
+        uint32_t va[8], vb[8], cw[CH];
+        load_group_col(rows, 0, col, va);
+        for (int w = 0; w < WORDS; w++) {
+            const uint32_t mine = ct_word(my_ct, w);
+#pragma unroll
+            for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
+            const int g = 4 * w;
+            if (g + 1 < NGROUP) load_group_col(rows, g + 1, col, vb);
+            add_group_col<0>(va, cw, acc, tab, lane);
+            if (g + 2 < NGROUP) load_group_col(rows, g + 2, col, va);
+            if (g + 1 < NGROUP) add_group_col<8>(vb, cw, acc, tab, lane);
+            if (g + 2 >= NGROUP) break;
+            load_group_col(rows, g + 3, col, vb);
+            add_group_col<16>(va, cw, acc, tab, lane);
+            if (g + 4 < NGROUP) load_group_col(rows, g + 4, col, va);
+            add_group_col<24>(vb, cw, acc, tab, lane);
+        }
+    }
+
+    static __device__ __forceinline__ void syndrome_pair(const FusedArgs &a, const FusedTile &tl, uint32_t *lds,
+                                                         int lane, int wv)
+    {
+        const int col = lane + 64 * wv;                // this wave's table column
+        const bool has_col = R == 2 || wv == 0;        // t <= 64: wave 0 owns every column
+        const bool live = lane < CH && lane < tl.count;
+        const int my_key = live ? a.item_key[tl.first + lane] : -1;
+        const uint8_t *ct_ptr = live ? a.ct + (size_t)a.items[tl.first + lane] * a.ct_stride : nullptr;
+        const int n_live = tl.count < CH ? tl.count : CH;
+        int runs = 0;
+        for (uint64_t rem = __ballot(live); rem; runs++) {
+            const int k = __builtin_amdgcn_readlane(my_key, __builtin_ctzll(rem));
+            rem &= ~__ballot(my_key == k);
+        }
+        uint32_t acc[CH];
+#pragma unroll
+        for (int x = 0; x < CH; x++) acc[x] = 0;
+        if (has_col && tl.count > SCALAR_BM && 2 * runs <= n_live) {
+            uint32_t *tab = lds + wv * 16 * 64;
+            tab[lane] = 0;  // entry 0
+            for (uint64_t rem = __ballot(live); rem;) {
+                const int key = __builtin_amdgcn_readlane(my_key, __builtin_ctzll(rem));
+                const bool in_run = my_key == key;
+                rem &= ~__ballot(in_run);
+                const KeyDev *kd = &a.keys[key];
+                stream_col(gptr(reinterpret_cast<const uint32_t *>(kd->table)), in_run ? ct_ptr : nullptr, col, acc,
+                           tab, lane);
+            }
+        } else if (has_col) {
+            // one key per ciphertext: only the rows of its set bits (the ciphertext is public),
+            // GRP at a time to keep loads in flight
+            constexpr int GRP = 16;
+#pragma unroll
+            for (int x = 0; x < CH; x++) {
+                if (x >= tl.count) continue;
+                const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
+                gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
+                const uint8_t *ctp = a.ct + (size_t)a.items[tl.first + x] * a.ct_stride;
+                for (int w = 0; w < WORDS; w++) {
+                    uint32_t bits = uni(ct_word(ctp, w));
+                    gu32 *wrows = rows + (size_t)(w * 32) * PAIRS;
+                    while (bits) {
+                        uint32_t v[GRP];
+#pragma unroll
+                        for (int j = 0; j < GRP; j++) {
+                            const bool any = bits != 0;
+                            const int bit = any ? __builtin_ctz(bits) : 0;
+                            bits &= bits - 1;
+                            v[j] = (any && col < PAIRS) ? wrows[(size_t)bit * PAIRS + col] : 0u;
+                        }
+#pragma unroll
+                        for (int j = 0; j < GRP; j++) acc[x] ^= v[j];
+                    }
+                }
+            }
+        }
+        __syncthreads();  // both waves' tables done with: the planes may overwrite them
+        if (wv == 0 && lane < kRowWords) lds[lane] = 0;  // row 0: the zero row read for s_{negative}
+        if (has_col && col < PAIRS) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int j = 2 * col + h;
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    uint32_t plane = 0;
+#pragma unroll
+                    for (int x = 0; x < CH; x++) plane |= ((acc[x] >> (16 * h + k)) & 1u) << x;
+                    lds[(j + 1) * kRowWords + k] = plane;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    template <int W, int LO, int HI>
+    static __device__ __forceinline__ void bm_step_pair(int n, const uint4 *rows4, uint32_t *xch, int lane,
+                                                        uint32_t (&C)[M], uint32_t (&B)[M], uint32_t (&bb)[M],
+                                                        uint32_t (&Lp)[9], uint32_t &ct, uint32_t &bt,
+                                                        uint32_t keep_last, uint32_t pm)
+    {
+        constexpr bool FULL = LO >= NFULL;
+        constexpr bool BLOCK = W < NBV;
+        constexpr bool TOP = TOPX && W == 1 && FULL;
+        // inside every phase a block's part in the products and in the shift is fixed
+        constexpr bool MAC = BLOCK && 64 * W <= LO + 1;
+        constexpr bool SHIFT = BLOCK && 64 * W <= LO + 2;
+        static_assert(!BLOCK || 64 * W <= LO + 1 || 64 * W > HI, "products join at a phase boundary");
+        static_assert(!BLOCK || 64 * W <= LO + 2 || 64 * W > HI + 1, "shift joins at a phase boundary");
+        uint32_t dp[M];
+        if constexpr (MAC) {
+            uint32_t p[2 * M - 1];
+#pragma unroll
+            for (int s2 = 0; s2 < 2 * M - 1; s2++) p[s2] = 0;
+            const int i = lane + 64 * W;
+            const int row = (n - i + 1) > 0 ? (n - i + 1) : 0;
+            uint32_t sv[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 w = rows4[row * 4 + q];
+                sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
+            }
+            bs_mac_wide<M>(C, sv, p);
+            uint32_t part[M];
+            bs_fold<M>(p, part);
+#pragma unroll
+            for (int k = 0; k < M; k++) dp[k] = wave_xor_all(part[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < M; k++) dp[k] = 0;
+        }
+        if constexpr (TOP) {
+            const int row = n - T + 1 > 0 ? n - T + 1 : 0;
+            uint32_t sv[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 w = rows4[row * 4 + q];
+                sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
+            }
+            uint32_t top = and_xor(0u, ct, uni(sv[M - 1]));
+#pragma unroll
+            for (int q = M - 2; q >= 0; q--) top = and_xor(tx_mulx(top, pm), ct, uni(sv[q]));
+#pragma unroll
+            for (int k = 0; k < M; k++) dp[k] ^= (uint32_t)__builtin_amdgcn_readlane((int)top, k);
+        }
+        // exchange through LDS
+        uint32_t *slot = xch + (n & 1) * XCH;
+        if (lane == 0) {
+            uint32_t w16[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) w16[k] = k < M ? dp[k] : 0u;
+            uint4 *dst = reinterpret_cast<uint4 *>(slot + (W == 0 ? 0 : 48));
+#pragma unroll
+            for (int q = 0; q < 4; q++) dst[q] = make_uint4(w16[4 * q], w16[4 * q + 1], w16[4 * q + 2], w16[4 * q + 3]);
+        }
+        if constexpr (W == 0) {
+            if (lane == 63) {
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    slot[16 + k] = C[k];
+                    slot[32 + k] = B[k];
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t d[M];
+        {
+            const uint4 *oth = reinterpret_cast<const uint4 *>(slot + (W == 0 ? 48 : 0));
+            uint32_t o[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 w = oth[q];
+                o[4 * q] = w.x; o[4 * q + 1] = w.y; o[4 * q + 2] = w.z; o[4 * q + 3] = w.w;
+            }
+#pragma unroll
+            for (int k = 0; k < M; k++) d[k] = dp[k] ^ uni(o[k]);
+        }
+        [[maybe_unused]] uint32_t c63[M], b63[M];
+        if constexpr (W == 1) {
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                c63[k] = slot[16 + k];
+                b63[k] = slot[32 + k];
+            }
+        }
+        uint32_t mne = 0;
+#pragma unroll
+        for (int k = 0; k < M; k++) mne |= d[k];
+        const uint32_t mle = mne & bs_le_const(Lp, (uint32_t)n >> 1);
+
+        if constexpr (TOP) {
+            // C_t' = b C_t + d B_t
+            uint32_t acc = and_xor(and_xor(0u, ct, bb[M - 1]), bt, d[M - 1]);
+#pragma unroll
+            for (int q = M - 2; q >= 0; q--) acc = and_xor(and_xor(tx_mulx(acc, pm), ct, bb[q]), bt, d[q]);
+            ct = acc;
+        }
+        if constexpr (MAC) {
+            uint32_t cn[M];
+            bs_mul_add2<M>(bb, C, d, B, cn);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                B[k] = __builtin_amdgcn_bitop3_b32(mle, C[k], B[k], 0xCA);  // mle ? C : B
+                C[k] = cn[k];
+            }
+        }
+        [[maybe_unused]] uint32_t carry[M];
+        if constexpr (W == 1) {
+            // lane 63 of wave 0's B after the selection: the coefficient its shift moves up
+#pragma unroll
+            for (int k = 0; k < M; k++) carry[k] = __builtin_amdgcn_bitop3_b32(mle, c63[k], b63[k], 0xCA);
+        }
+        if constexpr (TOP) {
+            // the shift moves coefficient t-1 of the selected B into B_t
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                uint32_t c;
+                if constexpr (NBV == 2)
+                    c = (uint32_t)__builtin_amdgcn_readlane((int)B[k], 63);
+                else
+                    c = carry[k];
+                bt = lane == k ? c : bt;
+            }
+        }
+        if constexpr (SHIFT) {
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                if constexpr (W == 0)
+                    B[k] = shift_up1_zero(B[k]);
+                else
+                    B[k] = shift_up1(B[k], carry[k]);
+            }
+        }
+        if constexpr (!TOPX && BLOCK && W == NBV - 1) {
+#pragma unroll
+            for (int k = 0; k < M; k++) B[k] &= keep_last;
+        }
+        uint32_t Ln[9];
+        bs_const_sub((uint32_t)n + 1, Lp, Ln);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Lp[q] = (Ln[q] & mle) | (Lp[q] & ~mle);
+#pragma unroll
+        for (int k = 0; k < M; k++) bb[k] = (d[k] & mle) | (bb[k] & ~mle);
+    }
+
+    // Wave W's part of the recurrence; leaves its coefficient rows (and the top row) in LDS.
+    template <int W>
+    static __device__ __forceinline__ void berlekamp_massey_pair(uint32_t *lds, uint32_t *xch, int lane)
+    {
+        uint32_t C[M], B[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            C[k] = (W == 0 && k == 0 && lane == 0) ? ~0u : 0u;  // C = 1
+            B[k] = (W == 0 && k == 0 && lane == 1) ? ~0u : 0u;  // B = x
+        }
+        uint32_t bb[M], Lp[9];
+#pragma unroll
+        for (int k = 0; k < M; k++) bb[k] = k == 0 ? ~0u : 0u;
+#pragma unroll
+        for (int q = 0; q < 9; q++) Lp[q] = 0;
+        const uint32_t keep_last = (lane + 64 * (NBV - 1) <= T) ? ~0u : 0u;
+        const uint32_t pm = (lane < M && ((Modulus<M>::low >> lane) & 1u)) ? ~0u : 0u;
+        uint32_t ct = 0, bt = 0;
+        const uint4 *rows4 = reinterpret_cast<const uint4 *>(lds);
+        static_assert(NBV <= 2, "two waves cover two lane blocks");
+        constexpr int P1 = NBV > 1 ? 62 : NFULL, P2 = NBV > 1 ? 63 : NFULL;
+#pragma unroll 1
+        for (int n = 0; n < P1; n++) bm_step_pair<W, 0, P1>(n, rows4, xch, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+#pragma unroll 1
+        for (int n = P1; n < P2; n++) bm_step_pair<W, P1, P2>(n, rows4, xch, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+#pragma unroll 1
+        for (int n = P2; n < NFULL; n++)
+            bm_step_pair<W, P2, NFULL>(n, rows4, xch, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+#pragma unroll 1
+        for (int n = NFULL; n < 2 * T; n++)
+            bm_step_pair<W, NFULL, 2 * T>(n, rows4, xch, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+        // every syndrome read of both waves finished before the last exchange barrier: the
+        // rows can now hold the coefficients
+        if constexpr (W < NBV) {
+            const int i = lane + 64 * W;
+            if (i <= T) {
+#pragma unroll
+                for (int k = 0; k < M; k++) lds[i * kRowWords + k] = C[k];
+            }
+        }
+        if constexpr (TOPX && W == 1) {
+            if (lane < M) lds[T * kRowWords + lane] = ct;  // lane k holds plane k of C_t
+        }
+    }
+
+    static __device__ __forceinline__ void run_pair(const FusedArgs &a, const FusedTile &tl, uint32_t *lds,
+                                                    uint32_t *xch, int lane, int wv, bool e_aligned)
+    {
+        static_assert(ALG == 1, "two-wavefront tiles use the additive-FFT root search");
+        syndrome_pair(a, tl, lds, lane, wv);
+        if (tl.count <= SCALAR_BM) {
+            if (wv == 0) {
+                uint32_t C[NBL][M];
+                berlekamp_massey_scalar(lds, lane, tl.count, C);
+                wave_sync();
+                write_coef_planes(C, lds, lane);
+            }
+        } else if (wv == 0) {
+            berlekamp_massey_pair<0>(lds, xch, lane);
+        } else {
+            berlekamp_massey_pair<1>(lds, xch, lane);
+        }
+        __syncthreads();
+        const FftTab *tab = a.fft + fft_variant(M, LOG);
+        if (wv == 0) fft_coeff_from_rows(lds, lane, tab);
+        __syncthreads();
+        roots_fft(a, tl, lds, lane, e_aligned, tab, wv, 2);
+    }
 };
+
+// LDS of a two-wavefront workgroup: the tile's planes/tables/rows/leaves, two exchange slots
+// and the tile-id slot.
+constexpr int kPairXchWords = 128;
+constexpr int kPairLds = kWaveLds + kPairXchWords + 16;
+static_assert(kWaveLds % 4 == 0, "exchange slots 16-byte aligned");
 
 
 // Persistent kernel: each wavefront pulls tiles until the list is exhausted.  Dynamic LDS of
@@ -1255,6 +1642,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
         default: break;
         }
     }
+}
+
+// Persistent kernel, two wavefronts per tile: each workgroup is one pair, which takes tiles
+// from the counter (wave 0 fetches, the id goes through LDS) until the list is exhausted.
+// OCC is the number of wavefronts per SIMD the build is register-limited for.
+template <int ALG, int OCC>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(OCC, OCC))) void k_fused_pair(FusedArgs a)
+{
+    extern __shared__ uint32_t lds_dyn[];
+    // the wave index as a wave-uniform value: branches on it (which hold barriers) stay
+    // scalar branches, never executed on both sides under an exec mask
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint32_t *xch = lds_dyn + kWaveLds;
+    uint32_t *tid_slot = xch + kPairXchWords;
+    const bool e_aligned = ((a.e_stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(a.e) & 3u) == 0);
+    for (;;) {
+        if (threadIdx.x == 0) *tid_slot = atomicAdd(a.tile_counter, 1u);
+        __syncthreads();
+        const uint32_t tid = uni(*tid_slot);
+        if (tid >= (uint32_t)a.n_tiles) break;  // both waves read the same id
+        const FusedTile tl = a.tiles[tid];
+        switch (tl.pidx) {
+        case 0: FusedTileKernel<0, ALG>::run_pair(a, tl, lds_dyn, xch, lane, wv, e_aligned); break;
+        case 1: FusedTileKernel<1, ALG>::run_pair(a, tl, lds_dyn, xch, lane, wv, e_aligned); break;
+        case 2: FusedTileKernel<2, ALG>::run_pair(a, tl, lds_dyn, xch, lane, wv, e_aligned); break;
+        case 3: FusedTileKernel<3, ALG>::run_pair(a, tl, lds_dyn, xch, lane, wv, e_aligned); break;
+        case 4: FusedTileKernel<4, ALG>::run_pair(a, tl, lds_dyn, xch, lane, wv, e_aligned); break;
+        default: break;
+        }
+        __syncthreads();  // the tile's LDS and the id slot are free again
+    }
+}
+
+template <int ALG, int OCC>
+static hipError_t launch_fused_pair_variant(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
+{
+    int groups = n_cu * s.groups_per_cu;
+    if (groups > a.n_tiles) groups = a.n_tiles;  // one tile per pair at most: spread small launches
+    if (groups < 1) groups = 1;
+    hipLaunchKernelGGL((k_fused_pair<ALG, OCC>), dim3(groups), dim3(128), (size_t)kPairLds * 4, st, a);
+    return hipGetLastError();
 }
 
 // Launches k_fused<ALG, OCC> with the schedule's workgroup size and persistent grid.

@@ -9,6 +9,7 @@ hipError_t launch_fused_fft_occ1(const FusedArgs &a, const FusedSchedule &s, int
 hipError_t launch_fused_fft_occ2(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
 hipError_t launch_fused_horner_occ1(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
 hipError_t launch_fused_horner_occ2(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
+hipError_t launch_fused_fft_pair(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st);
 
 // Zero fill used instead of hipMemsetAsync so the launch sequence can be graph-captured.
 __global__ void k_zero_u32(uint32_t *p, int n)
@@ -29,13 +30,17 @@ bool fused_supported(const FusedSchedule &s)
     // small batch over more wavefronts
     const bool tile_ok = s.tile_cts >= 1 && s.tile_cts <= 32 && (s.tile_cts & (s.tile_cts - 1)) == 0;
     const bool wpg_ok = s.waves_per_group == 1 || s.waves_per_group == 2 || s.waves_per_group == 4;
-    return tile_ok && wpg_ok && s.waves_per_ct == 1 && (s.root_alg == 0 || s.root_alg == 1);
+    const bool alg_ok = s.root_alg == 0 || s.root_alg == 1;
+    if (s.waves_per_ct == 2)  // one pair per workgroup, additive-FFT root search
+        return tile_ok && s.waves_per_group == 2 && s.root_alg == 1;
+    return tile_ok && wpg_ok && s.waves_per_ct == 1 && alg_ok;
 }
 
 hipError_t launch_fused(const FusedArgs &a, const FusedSchedule &s, int n_cu, hipStream_t st)
 {
     if (!fused_supported(s)) return hipErrorInvalidValue;
     if (a.n_tiles <= 0) return hipSuccess;
+    if (s.waves_per_ct == 2) return launch_fused_fft_pair(a, s, n_cu, st);
     // more than one wavefront per SIMD needs the 256-register build
     const bool occ2 = s.waves_per_group * s.groups_per_cu > 4;
     // Fewer tiles than resident wavefronts (small and serving launches): one wavefront per
