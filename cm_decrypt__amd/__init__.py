@@ -130,6 +130,7 @@ _sig("cmd_errorstream_write", _I, _I32, _c.c_char_p, _P, _SZ, _SZ, _I)
 _sig("cmd_gen_keypair", _I, _I32, _U64, _P)
 _sig("cmd_gen_encrypt", _I, _I32, _P, _SZ, _U64, _SZ, _I32, _P, _P)
 _sig("cmd_gen_keys", _I, _I32, _U64, _SZ, _P, _I)
+_sig("cmd_gen_keys_device", _I, _P, _I32, _U64, _SZ, _P, _I)
 _sig("cmd_controlbits", _I, _I32, _P, _P)
 _sig("cmd_ctx_create", _I, _I32, _U64, _c.POINTER(_P))
 _sig("cmd_ctx_destroy", _I, _P)
@@ -433,6 +434,14 @@ class Context:
         _check(_lib.cmd_gen_encrypt_device(self._h, keys.size, _ptr(keys), _ptr(sd), per_key, weight, _ptr(ct),
                                            _ptr(e)), "cmd_gen_encrypt_device")
         return ct, e
+
+    def gen_keys(self, param_id: int, base_seed: int, n_keys: int, threads: int = 8) -> np.ndarray:
+        """The key pairs of gen_keys (same bytes), with each attempt's systematic-form test on the GPU."""
+        p = params(param_id)
+        out = np.zeros((n_keys, p.sk_bytes), np.uint8)
+        _check(_lib.cmd_gen_keys_device(self._h, param_id, base_seed, n_keys, _ptr(out), threads),
+               "cmd_gen_keys_device")
+        return out
 
     def host_array(self, shape, dtype=np.uint8) -> np.ndarray:
         """A numpy array in pinned host memory (cmd_host_alloc): decrypt() moves such buffers by

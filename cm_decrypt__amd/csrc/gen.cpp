@@ -300,40 +300,71 @@ static void encrypt_one(const Encryptor &E, Rng &rng, int weight, uint8_t *ct, u
 
 // ---- key pairs -------------------------------------------------------------------------
 
-// One attempt per seed value; returns false when this seed gives no usable key.
-static bool keypair_attempt(int pidx, uint64_t seed, uint8_t *sk)
+// The randomness of one key-pair attempt up to its support: the Goppa polynomial (minimal
+// polynomial of a random element of the extension) and the support permutation, with the
+// random stream positioned after the permutation for the key's remaining bytes.
+struct KeyCandidate {
+    Rng rng{0};
+    std::vector<uint16_t> g;  // t+1 coefficients, monic
+    std::vector<int> pi;      // permutation of 0 .. 2^m - 1; L_i = bitrev(pi_i)
+    bool poly_ok = false;     // false: the element's powers are dependent, no key from this seed
+};
+
+static void draw_candidate(int pidx, uint64_t seed, KeyCandidate &c)
 {
     const ParamDesc &p = kParams[pidx];
     HostField F(p.m);
-    Rng rng(seed ^ ((uint64_t)p.id << 32));
-    std::vector<uint16_t> f(p.t), g(p.t + 1);
-    for (int i = 0; i < p.t; i++) f[i] = (uint16_t)(rng.next() & F.mask);
-    if (!minimal_poly(pidx, f.data(), g.data())) return false;
-    g[p.t] = 1;
+    c.rng = Rng(seed ^ ((uint64_t)p.id << 32));
+    std::vector<uint16_t> f(p.t);
+    c.g.assign(p.t + 1, 0);
+    for (int i = 0; i < p.t; i++) f[i] = (uint16_t)(c.rng.next() & F.mask);
+    c.poly_ok = minimal_poly(pidx, f.data(), c.g.data());
+    if (!c.poly_ok) return;
+    c.g[p.t] = 1;
     const int q = 1 << p.m;
-    std::vector<int> pi(q);
-    for (int i = 0; i < q; i++) pi[i] = i;
-    for (int i = q - 1; i > 0; i--) std::swap(pi[i], pi[rng.below((uint32_t)(i + 1))]);
-    std::vector<uint8_t> cond(p.cond_bytes);
-    if (!controlbits(p.m, pi, cond.data(), cond.size())) return false;
-    std::vector<uint16_t> L(p.n);
-    support_gen(pidx, cond.data(), L.data());
-    Encryptor E;
-    if (!encryptor_init(pidx, g.data(), L.data(), E)) return false;
+    c.pi.resize(q);
+    for (int i = 0; i < q; i++) c.pi[i] = i;
+    for (int i = q - 1; i > 0; i--) std::swap(c.pi[i], c.pi[c.rng.below((uint32_t)(i + 1))]);
+}
 
+// The secret key of an accepted candidate (operations.c:62-101 layout): 32 random bytes,
+// the 8-byte pivot prefix, g, the condition bits, then n/8 random bytes of s.
+static void write_sk(int pidx, KeyCandidate &c, const uint8_t *cond, uint8_t *sk)
+{
+    const ParamDesc &p = kParams[pidx];
     uint8_t *w = sk;
-    for (int i = 0; i < 32; i++) w[i] = (uint8_t)(rng.next() & 0xFF);
+    for (int i = 0; i < 32; i++) w[i] = (uint8_t)(c.rng.next() & 0xFF);
     w += 32;
     const uint8_t pivots[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0, 0, 0, 0};  // operations.c:101
     std::memcpy(w, pivots, 8);
     w += 8;
-    for (int i = 0; i < p.t; i++) { w[2 * i] = (uint8_t)(g[i] & 0xFF); w[2 * i + 1] = (uint8_t)(g[i] >> 8); }
+    for (int i = 0; i < p.t; i++) { w[2 * i] = (uint8_t)(c.g[i] & 0xFF); w[2 * i + 1] = (uint8_t)(c.g[i] >> 8); }
     w += 2 * p.t;
-    std::memcpy(w, cond.data(), cond.size());
-    w += cond.size();
-    for (int i = 0; i < p.n / 8; i++) w[i] = (uint8_t)(rng.next() & 0xFF);
+    std::memcpy(w, cond, (size_t)p.cond_bytes);
+    w += p.cond_bytes;
+    for (int i = 0; i < p.n / 8; i++) w[i] = (uint8_t)(c.rng.next() & 0xFF);
+}
+
+// One attempt per seed value; returns false when this seed gives no usable key.
+static bool keypair_attempt(int pidx, uint64_t seed, uint8_t *sk)
+{
+    const ParamDesc &p = kParams[pidx];
+    KeyCandidate c;
+    draw_candidate(pidx, seed, c);
+    if (!c.poly_ok) return false;
+    std::vector<uint8_t> cond(p.cond_bytes);
+    if (!controlbits(p.m, c.pi, cond.data(), cond.size())) return false;
+    std::vector<uint16_t> L(p.n);
+    support_gen(pidx, cond.data(), L.data());
+    Encryptor E;
+    if (!encryptor_init(pidx, c.g.data(), L.data(), E)) return false;
+    write_sk(pidx, c, cond.data(), sk);
     return true;
 }
+
+// Seed of attempt a of a key whose base seed is s (cmd_gen_keypair's sequence).
+static inline uint64_t attempt_seed(uint64_t s, int a) { return s + ((uint64_t)a << 40); }
+constexpr int kMaxAttempts = 64;
 
 }  // namespace cmd
 
@@ -345,8 +376,8 @@ extern "C" int cmd_gen_keypair(int32_t param_id, uint64_t seed, uint8_t *sk)
     if (pidx < 0) return CMD_ERR_PARAM;
     if (!sk) return CMD_ERR_ARG;
     // Deterministic: the first seed value s, s+2^40, s+2*2^40, ... that yields a key.
-    for (uint64_t k = 0; k < 64; k++)
-        if (keypair_attempt(pidx, seed + (k << 40), sk)) return CMD_OK;
+    for (int a = 0; a < kMaxAttempts; a++)
+        if (keypair_attempt(pidx, attempt_seed(seed, a), sk)) return CMD_OK;
     return CMD_ERR_SINGULAR;
 }
 
@@ -489,6 +520,10 @@ extern "C" int cmd_gen_encrypt_device(cmd_ctx *ctx, size_t n_keys, const int32_t
             a.weight = weight;
             a.pos = (const uint16_t *)d_pos.ptr;
             a.ct = (uint8_t *)d_ct.ptr;
+            a.cand_g = nullptr;
+            a.cand_L = nullptr;
+            a.rank_only = 0;
+            a.pad0 = 0;
             hipError_t he = hipMemcpyAsync(d_small.ptr, key_ids + k0, nk * 4, hipMemcpyHostToDevice, ctx->stream);
             if (he == hipSuccess)
                 he = hipMemcpyAsync((int32_t *)d_small.ptr + nk, flags.data(), nk * 4, hipMemcpyHostToDevice, ctx->stream);
@@ -509,5 +544,126 @@ extern "C" int cmd_gen_encrypt_device(cmd_ctx *ctx, size_t n_keys, const int32_t
             if (flags[k]) rc = set_error(CMD_ERR_SINGULAR, "key %zu has no systematic form", k0 + k);
         for (DevBuf *b : {&d_cols, &d_aug, &d_ainvt, &d_small, &d_pos, &d_ct}) dev_free(ctx, *b);
     }
+    return rc;
+}
+
+// ---- key pairs with the systematic-form test on the GPU ---------------------------------------
+//
+// The same keys as cmd_gen_keys (key k from base_seed + k, attempt a of a key from seed
+// base_seed + k + a 2^40, the first attempt that yields a key wins).  Per round, host threads
+// draw the next attempts of every unfinished key (Goppa polynomial and permutation: the host
+// generator's own code), the GPU tests all their A matrices at once (H's first mt columns from
+// g and L_i = bitrev(pi_i), Gauss-Jordan for the rank), and per key the first attempt, in seed
+// order, with a minimal polynomial, an invertible A and a routable permutation becomes the key.
+
+extern "C" int cmd_gen_keys_device(cmd_ctx *ctx, int32_t param_id, uint64_t base_seed, size_t n_keys, uint8_t *sk_out,
+                                   int threads)
+{
+    const int pidx = param_index(param_id);
+    if (pidx < 0) return set_error(CMD_ERR_PARAM, "unknown parameter set %d", (int)param_id);
+    if (!ctx || (n_keys && !sk_out)) return CMD_ERR_ARG;
+    if (!n_keys) return CMD_OK;
+    const ParamDesc &p = kParams[pidx];
+    if (threads < 1) threads = 1;
+    std::lock_guard<std::recursive_mutex> lock(ctx->lock);
+    CMD_HIP(hipSetDevice(ctx->device));
+    const size_t mt = (size_t)p.m * p.t, cw = (mt + 31) / 32;
+    const size_t skb = (size_t)p.sk_bytes;
+    constexpr int kSpec = 2;                   // attempts per unfinished key and round
+    const size_t per_cand = (size_t)p.n * cw * 4 + mt * 2 * cw * 4;
+    const size_t kBatch = std::max<size_t>(16, std::min<size_t>(512, (size_t)(2ull << 30) / per_cand));
+
+    std::vector<int> next_attempt(n_keys, 0);
+    std::vector<char> done(n_keys, 0);
+    DevBuf d_cols, d_aug, d_small, d_g, d_L;
+    int rc = CMD_OK;
+    for (size_t remaining = n_keys; remaining && !rc;) {
+        // this round's candidates: (key, attempt)
+        struct Cand { size_t key; int attempt; };
+        std::vector<Cand> cands;
+        for (size_t k = 0; k < n_keys; k++) {
+            if (done[k]) continue;
+            for (int s = 0; s < kSpec && next_attempt[k] + s < kMaxAttempts; s++) cands.push_back({k, next_attempt[k] + s});
+            if (next_attempt[k] >= kMaxAttempts) {
+                rc = set_error(CMD_ERR_SINGULAR, "key %zu: no key in %d attempts", k, kMaxAttempts);
+                break;
+            }
+        }
+        if (rc) break;
+        std::vector<KeyCandidate> kc(cands.size());
+        {
+            std::atomic<size_t> next{0};
+            auto work = [&]() {
+                for (size_t i; (i = next.fetch_add(1)) < cands.size();)
+                    draw_candidate(pidx, attempt_seed(base_seed + cands[i].key, cands[i].attempt), kc[i]);
+            };
+            std::vector<std::thread> pool;
+            for (int i = 1; i < threads && (size_t)i < cands.size(); i++) pool.emplace_back(work);
+            work();
+            for (auto &th : pool) th.join();
+        }
+        // the GPU's rank test for every candidate with a Goppa polynomial
+        std::vector<int32_t> singular(cands.size(), 1);
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < cands.size(); i++)
+            if (kc[i].poly_ok) idx.push_back(i);
+        for (size_t b0 = 0; b0 < idx.size() && !rc; b0 += kBatch) {
+            const size_t nb = std::min(kBatch, idx.size() - b0);
+            std::vector<uint16_t> hg(nb * p.t), hL(nb * (size_t)p.n);
+            const HostField F(p.m);
+            for (size_t q = 0; q < nb; q++) {
+                const KeyCandidate &c = kc[idx[b0 + q]];
+                std::memcpy(&hg[q * p.t], c.g.data(), sizeof(uint16_t) * p.t);
+                for (int i = 0; i < p.n; i++) hL[q * p.n + i] = F.bitrev((uint16_t)c.pi[i]);
+            }
+            rc = dev_reserve(ctx, d_cols, nb * (size_t)p.n * cw * 4);
+            if (!rc) rc = dev_reserve(ctx, d_aug, nb * mt * 2 * cw * 4);
+            if (!rc) rc = dev_reserve(ctx, d_small, nb * 4);
+            if (!rc) rc = dev_reserve(ctx, d_g, hg.size() * 2);
+            if (!rc) rc = dev_reserve(ctx, d_L, hL.size() * 2);
+            if (rc) break;
+            GenArgs a{};
+            a.pidx = pidx;
+            a.n_keys = (int32_t)nb;
+            a.keys = ctx->d_keys;
+            a.cols = (uint32_t *)d_cols.ptr;
+            a.aug = (uint32_t *)d_aug.ptr;
+            a.singular = (int32_t *)d_small.ptr;
+            a.cand_g = (const uint16_t *)d_g.ptr;
+            a.cand_L = (const uint16_t *)d_L.ptr;
+            a.rank_only = 1;
+            hipError_t he = hipMemcpyAsync(d_g.ptr, hg.data(), hg.size() * 2, hipMemcpyHostToDevice, ctx->stream);
+            if (he == hipSuccess) he = hipMemcpyAsync(d_L.ptr, hL.data(), hL.size() * 2, hipMemcpyHostToDevice, ctx->stream);
+            if (he == hipSuccess) he = launch_gen_check(a, ctx->stream);
+            std::vector<int32_t> flags(nb, 1);
+            if (he == hipSuccess)
+                he = hipMemcpyAsync(flags.data(), d_small.ptr, nb * 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+            if (he != hipSuccess) rc = hip_fail(he, "key-pair rank test");
+            for (size_t q = 0; q < nb && !rc; q++) singular[idx[b0 + q]] = flags[q];
+        }
+        if (rc) break;
+        // per key, the first winning attempt in seed order
+        std::vector<size_t> winner(n_keys, SIZE_MAX);
+        std::vector<std::vector<uint8_t>> conds(cands.size());
+        for (size_t i = 0; i < cands.size(); i++) {
+            const size_t k = cands[i].key;
+            if (winner[k] != SIZE_MAX || !kc[i].poly_ok || singular[i]) continue;
+            conds[i].assign(p.cond_bytes, 0);
+            if (controlbits(p.m, kc[i].pi, conds[i].data(), conds[i].size())) winner[k] = i;
+        }
+        for (size_t k = 0; k < n_keys; k++) {
+            if (done[k]) continue;
+            if (winner[k] != SIZE_MAX) {
+                const size_t i = winner[k];
+                write_sk(pidx, kc[i], conds[i].data(), sk_out + k * skb);
+                done[k] = 1;
+                remaining--;
+            } else {
+                next_attempt[k] += kSpec;
+            }
+        }
+    }
+    for (DevBuf *b : {&d_cols, &d_aug, &d_small, &d_g, &d_L}) dev_free(ctx, *b);
     return rc;
 }

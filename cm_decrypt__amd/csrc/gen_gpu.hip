@@ -9,7 +9,8 @@
 //   k_gen_aug     [A | I] as bit rows (32 x 32 bit transposes of the column words);
 //   k_gen_invert  Gauss-Jordan on [A | I] (one workgroup per key; pivot search, row swap and
 //                 elimination by whole wavefronts per row) leaving [I | A^-1]; a key whose A
-//                 is singular is flagged;
+//                 is singular is flagged (with rank_only, within A alone: key-pair candidates
+//                 need only that flag, the systematic-form test of pk_gen.c:35-149);
 //   k_gen_ainvt   A^-1 as columns.
 // Per ciphertext (one wavefront): y = H e as the XOR of the columns at its error positions
 // (lane = word), then ct = A^-1 y as the XOR of the columns of A^-1 at the set bits of y.
@@ -38,12 +39,18 @@ __global__ __launch_bounds__(256) void k_gen_cols(GenArgs a)
     constexpr uint32_t MASK = (1u << G::M) - 1u;
     const int j = blockIdx.x * 256 + threadIdx.x, ks = blockIdx.y;
     if (j >= G::N) return;
-    const KeyDev &kd = a.keys[a.key_ids[ks]];
-    const uint32_t x = kd.L[j];
-    uint32_t ev = 1;  // g(x), g_t = 1 leading
-    for (int i = G::T - 1; i >= 0; i--) {
-        const uint32_t gi = ((uint32_t)kd.gbytes[2 * i] | ((uint32_t)kd.gbytes[2 * i + 1] << 8)) & MASK;
-        ev = gf_mul<G::M>(ev, x) ^ gi;
+    uint32_t x, ev = 1;  // g(x), g_t = 1 leading
+    if (a.cand_L) {
+        const uint16_t *cg = a.cand_g + (size_t)ks * G::T;
+        x = a.cand_L[(size_t)ks * G::N + j];
+        for (int i = G::T - 1; i >= 0; i--) ev = gf_mul<G::M>(ev, x) ^ ((uint32_t)cg[i] & MASK);
+    } else {
+        const KeyDev &kd = a.keys[a.key_ids[ks]];
+        x = kd.L[j];
+        for (int i = G::T - 1; i >= 0; i--) {
+            const uint32_t gi = ((uint32_t)kd.gbytes[2 * i] | ((uint32_t)kd.gbytes[2 * i + 1] << 8)) & MASK;
+            ev = gf_mul<G::M>(ev, x) ^ gi;
+        }
     }
     uint32_t v = gf_inv<G::M>(ev);
     uint32_t *col = a.cols + ((size_t)ks * G::N + j) * G::CW;
@@ -113,6 +120,8 @@ __global__ __launch_bounds__(kInvThreads) void k_gen_invert(GenArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, ks = blockIdx.x;
     constexpr int WAVES = kInvThreads / 64;
     uint32_t *A = a.aug + (size_t)ks * G::MT * G::RW;
+    // words of a row taking part: all of [A | I], or only A when just its rank is wanted
+    const int RWU = a.rank_only ? G::CW : G::RW;
     for (int c = 0; c < G::MT; c++) {
         const int cw = c >> 5;
         const uint32_t cb = 1u << (c & 31);
@@ -131,21 +140,21 @@ __global__ __launch_bounds__(kInvThreads) void k_gen_invert(GenArgs a)
             return;
         }
         if (p != c) {
-            for (int w = tid; w < G::RW; w += kInvThreads) {
+            for (int w = tid; w < RWU; w += kInvThreads) {
                 const uint32_t t0 = A[(size_t)p * G::RW + w];
                 A[(size_t)p * G::RW + w] = A[(size_t)c * G::RW + w];
                 A[(size_t)c * G::RW + w] = t0;
             }
         }
         __syncthreads();
-        for (int w = tid; w < G::RW; w += kInvThreads) prow[w] = A[(size_t)c * G::RW + w];
+        for (int w = tid; w < RWU; w += kInvThreads) prow[w] = A[(size_t)c * G::RW + w];
         for (int r = tid; r < G::MT; r += kInvThreads)
             if (r != c && (A[(size_t)r * G::RW + cw] & cb)) atomicOr(&flag[r >> 5], 1u << (r & 31));
         __syncthreads();
         // rows with bit c set ^= pivot row, words cw .. RW-1 (those below cw are zero in it)
         for (int r = wave; r < G::MT; r += WAVES) {
             if (!((flag[r >> 5] >> (r & 31)) & 1u)) continue;
-            for (int w = cw + lane; w < G::RW; w += 64) A[(size_t)r * G::RW + w] ^= prow[w];
+            for (int w = cw + lane; w < RWU; w += 64) A[(size_t)r * G::RW + w] ^= prow[w];
         }
         __syncthreads();
     }
@@ -223,6 +232,16 @@ hipError_t gen_keys(const GenArgs &a, hipStream_t st)
 }
 
 template <int IDX>
+hipError_t gen_check(const GenArgs &a, hipStream_t st)
+{
+    using G = Gen<IDX>;
+    hipLaunchKernelGGL(k_gen_cols<IDX>, dim3((G::N + 255) / 256, a.n_keys), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_gen_aug<IDX>, dim3((G::CW * G::CW + 255) / 256, a.n_keys), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_gen_invert<IDX>, dim3(a.n_keys), dim3(kInvThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int IDX>
 hipError_t gen_encrypt(const GenArgs &a, hipStream_t st)
 {
     const int total = a.n_keys * a.per_key;
@@ -241,6 +260,19 @@ hipError_t launch_gen_keys(const GenArgs &a, hipStream_t st)
     case 2: return gen_keys<2>(a, st);
     case 3: return gen_keys<3>(a, st);
     case 4: return gen_keys<4>(a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_gen_check(const GenArgs &a, hipStream_t st)
+{
+    if (a.n_keys <= 0) return hipSuccess;
+    switch (a.pidx) {
+    case 0: return gen_check<0>(a, st);
+    case 1: return gen_check<1>(a, st);
+    case 2: return gen_check<2>(a, st);
+    case 3: return gen_check<3>(a, st);
+    case 4: return gen_check<4>(a, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -153,8 +153,15 @@ struct GenArgs {
     int32_t weight;           // error positions per ciphertext
     const uint16_t *pos;      // per ciphertext: `weight` positions (key-major order)
     uint8_t *ct;              // per ciphertext: synd_bytes
+    // key-pair candidates instead of resident keys (both non-null): per candidate t Goppa
+    // coefficients (g_t = 1 implied) and n support elements
+    const uint16_t *cand_g;
+    const uint16_t *cand_L;
+    int32_t rank_only;        // 1: eliminate within A only (its invertibility), not A^-1
+    int32_t pad0;
 };
 hipError_t launch_gen_keys(const GenArgs &a, hipStream_t st);     // columns and inverses
+hipError_t launch_gen_check(const GenArgs &a, hipStream_t st);    // columns and invertibility of A
 hipError_t launch_gen_encrypt(const GenArgs &a, hipStream_t st);  // ciphertexts
 
 // Host: fills the FFT constants for (m, l).

@@ -151,6 +151,15 @@ int cmd_key_export(cmd_ctx *ctx, int32_t key_id, uint16_t *table, uint16_t *L, u
 /* Device bytes one resident key of this set occupies. */
 int cmd_key_device_bytes(int32_t param_id, uint64_t *bytes);
 
+/* The same key pairs as cmd_gen_keys (byte for byte), with the systematic-form test of every
+ * attempt on the GPU: host threads draw each attempt's Goppa polynomial and support
+ * permutation, the GPU builds the first mt parity-check columns of all attempts of a round and
+ * decides their invertibility by Gauss-Jordan (the work of the reference's pk_gen.c:35-149 in
+ * crypto_kem_keypair, kem/common/core/operations.c:36-107), and each key takes its first
+ * successful attempt in seed order. */
+int cmd_gen_keys_device(cmd_ctx *ctx, int32_t param_id, uint64_t base_seed, size_t n_keys, uint8_t *sk_out,
+                        int threads);
+
 /* The same ciphertexts as cmd_gen_encrypt, for resident keys of one set, computed on the GPU:
  * ct rows k*per_key .. (k+1)*per_key-1 (SYND_BYTES each) are cmd_gen_encrypt(param, sk of
  * key_ids[k], seeds[k], per_key, weight) and e (optional, n/8 bytes per row) its errors.
@@ -236,16 +245,19 @@ int cmd_stage_synd_full(cmd_ctx *ctx, int32_t key_id, size_t count, const uint8_
                         uint16_t *s_out);
 
 /* ---- schedules (autotuner) ----------------------------------------------------------- */
-/* Fused-kernel schedule per set: ciphertexts per wavefront tile (16/32), waves per workgroup,
- * waves per ciphertext in the root stage, persistent groups per CU, and the root-search
- * algorithm (0: Horner at each support element as root_13.c:216-240 does; 1: additive FFT
- * over all 2^m field elements followed by the key's Benes network).  Both give the same
- * error vectors; the autotuner measures candidates on the GPU. */
+/* Fused-kernel schedule per set: ciphertexts per tile, wavefronts per workgroup, wavefronts
+ * per tile, persistent groups per CU, and the root-search algorithm (0: Horner at each support
+ * element as root_13.c:216-240 does; 1: additive FFT over all 2^m field elements followed by
+ * the key's Benes network).  All give the same error vectors; cmd_autotune measures candidates
+ * on the GPU.  A launch over several sets takes the wavefront layout of its heaviest set and
+ * each set's own tile width. */
 typedef struct {
-    int32_t tile_cts;        /* ciphertexts per wavefront tile: 1, 2, 4, 8, 16 or 32 (the kernel is
+    int32_t tile_cts;        /* ciphertexts per tile: 1, 2, 4, 8, 16 or 32 (the kernel is
                                 bit-sliced 32 wide; narrower tiles spread small batches wider) */
-    int32_t waves_per_group; /* wavefronts per workgroup */
-    int32_t waves_per_ct;    /* wavefronts sharing one ciphertext's root search */
+    int32_t waves_per_group; /* wavefronts per workgroup: 1, 2 or 4 (2 when waves_per_ct is 2) */
+    int32_t waves_per_ct;    /* wavefronts per tile: 1, or 2 (the tile's syndrome columns and
+                                Berlekamp-Massey lane blocks split between a pair of wavefronts,
+                                four wavefronts per SIMD; additive-FFT root search only) */
     int32_t groups_per_cu;   /* persistent workgroups per compute unit */
     int32_t root_alg;        /* 0 Horner, 1 additive FFT + Benes */
 } cmd_schedule;

@@ -447,3 +447,22 @@ def test_gpu_generator_matches_host_generator(ctx, pid):
     finally:
         for k in kids:
             ctx.unload_key(k)
+
+
+@pytest.mark.parametrize("pid", PIDS)
+def test_gpu_keypairs_match_host_generator(ctx, pid):
+    """cmd_gen_keys_device: the systematic-form test of every attempt on the GPU gives the host
+    generator's secret keys byte for byte (about 3.5 attempts per key: random binary matrices
+    are invertible with probability about 0.29, so redraws are exercised), and the keys load
+    and decrypt."""
+    n = 6
+    want = cm.gen_keys(pid, 3300 + pid % 1000, n, threads=8)
+    print(f"device keypairs pid={pid} keys={n}", flush=True)
+    got = ctx.gen_keys(pid, 3300 + pid % 1000, n, threads=8)
+    assert np.array_equal(got, want)
+    kid = ctx.load_key(pid, got[-1])
+    try:
+        ct, e = cm.gen_encrypt(pid, got[-1], 5, 4)
+        assert np.array_equal(ctx.decrypt([kid] * len(ct), ct)[:, :e.shape[1]], e)
+    finally:
+        ctx.unload_key(kid)
