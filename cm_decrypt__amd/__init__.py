@@ -26,7 +26,9 @@ from typing import Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcmdecrypt.so")
+# CMD_DECRYPT_LIB selects another build of the same library (the sanitizer build of the host
+# tests); it must export the same C ABI.
+LIB_PATH = os.environ.get("CMD_DECRYPT_LIB") or os.path.join(_HERE, "libcmdecrypt.so")
 
 PARAM_IDS = (348864, 460896, 6688128, 6960119, 8192128)
 MODE_PLAIN = 0
@@ -63,10 +65,11 @@ class CmdError(RuntimeError):
 def _load():
     # torch (when installed) bundles its own HIP runtime under the same soname; loading it
     # first makes this library bind to that one runtime instead of a second copy.
-    try:
-        import torch  # noqa: F401
-    except Exception:
-        pass
+    if not os.environ.get("CMD_DECRYPT_NO_TORCH"):
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (or __graft_entry__.build())")
