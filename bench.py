@@ -425,6 +425,57 @@ def cpu_baseline(cm, sks, ct, key_of, budget_s):
     return out, all_cores
 
 
+def tune_all(cm, torch, ctx, path, threads):
+    """Autotunes every set on a launch-filling batch (16 keys x 65536 ciphertexts of that set)
+    and writes the schedules to path."""
+    out = {}
+    for pid, _ in MIX:
+        sks = cm.gen_keys(pid, 4242 + pid % 1000, N_KEYS, threads=threads)
+        kids = [ctx.load_key(pid, s) for s in sks]
+        key_ids = np.array([kids[i % N_KEYS] for i in range(N_CT)], np.int32)
+        best, us = ctx.autotune(pid, key_ids, reps=3)
+        out[str(pid)] = {"schedule": best, "us": round(us, 1)}
+        log(f"tuned {pid}: {best} ({us:.0f} us for {N_CT} items)")
+        for k in kids:
+            ctx.unload_key(k)
+    ctx.save_schedules(path)
+    return out
+
+
+def key_independence(cm, torch, ctx, sks, ct, key_of):
+    """The same 8192 ciphertexts decrypted under two different key assignments (the second
+    decodes non-codewords), on the bit-sliced path (32-ciphertext tiles) and on the scalar
+    one-ciphertext path: four fused launches in the order (32, A), (32, B), (1, A), (1, B).
+    Under rocprofv3 --pmc their instruction counts must agree pairwise: the decoder's work does
+    not depend on the secret key or the error vector."""
+    n = 8192
+    kids = [ctx.load_key(SET, s) for s in sks]
+    d_ct = torch.from_numpy(np.ascontiguousarray(ct[:n])).cuda()
+    d_e = torch.zeros((n, 1024), dtype=torch.uint8, device="cuda")
+    a = np.array([kids[k] for k in key_of[:n]], np.int32)
+    b = np.array([kids[(k + 5) % len(kids)] for k in key_of[:n]], np.int32)
+    saved = ctx.schedule(SET)
+    ctx.set_tile_fit(False)
+    res = []
+    try:
+        for tile in (32, 1):
+            ctx.set_schedule(SET, tile_cts=tile)
+            for name, keys in (("A", a), ("B", b)):
+                d_e.zero_()
+                torch.cuda.synchronize()
+                ctx.decrypt_device(keys, d_ct.data_ptr(), 208, d_e.data_ptr(), 1024)
+                torch.cuda.synchronize()
+                digest = hashlib.sha256(d_e.cpu().numpy().tobytes()).hexdigest()[:16]
+                res.append({"tile": tile, "keys": name, "e_digest": digest})
+                log(f"key independence: tile {tile}, keys {name}, e digest {digest}")
+    finally:
+        ctx.set_tile_fit(True)
+        ctx.set_schedule(SET, **saved)
+        for k in kids:
+            ctx.unload_key(k)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -452,6 +503,11 @@ def main():
     ap.add_argument("--schedules", default=os.path.join(ROOT, "cm_decrypt__amd", "schedules.json"),
                     help="tuned schedules (the committed profiles were taken with this file)")
     ap.add_argument("--save-schedules", default="", help="write the autotuned schedules here")
+    ap.add_argument("--tune-all", default="",
+                    help="autotune every set on a launch-filling batch, write the schedules here and exit")
+    ap.add_argument("--key-independence", action="store_true",
+                    help="four fused launches of the same ciphertexts under two key assignments "
+                         "(profile them with rocprofv3 --pmc) and exit")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -488,6 +544,16 @@ def main():
         log(f"copy probe: {args.copy_probe_mb} MiB device-to-device")
         del src, dst
     stream = torch.cuda.Stream()  # default priority; graphs capture on this stream
+    if args.tune_all:
+        res = tune_all(cm, torch, ctx, args.tune_all, threads)
+        print(json.dumps({"tuned": res, "written": args.tune_all}), flush=True)
+        return
+    if args.key_independence:
+        if os.path.exists(args.schedules):
+            ctx.load_schedules(args.schedules)
+        res = key_independence(cm, torch, ctx, sks, ct, key_of)
+        print(json.dumps({"key_independence": res}), flush=True)
+        return
     tuned = None
     if args.autotune:
         best, best_us = ctx.autotune(SET, key_ids, reps=2)
