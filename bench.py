@@ -24,6 +24,13 @@ import time
 
 import numpy as np
 
+# Hardware queues of the HIP runtime for this process (read once, when the runtime starts, so
+# set before anything touches the GPU).  The runtime's default of 4 is shared by every stream
+# of the process; the serving loop's launch slots then land on a queue another slot (or the
+# headline leg's stream) already uses and wait behind its launch.  16 gives each stream a queue
+# of its own; a value in the environment takes precedence.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -263,6 +270,25 @@ def mixed_leg(cm, torch, ctx, pool, n_items, keys_per_set, steps, seed, threads)
             "mix": {str(pid): counts[pid] for pid, _ in MIX}, "schedules": tuned}
 
 
+def cpu_sched_snapshot():
+    """The process's involuntary context switches and its cgroup's CPU-quota throttling
+    counters (cgroup v2 or v1, where readable): host-side evidence for stalls of the serving
+    threads that no GPU event explains."""
+    import resource
+    ru = resource.getrusage(resource.RUSAGE_SELF)
+    snap = {"involuntary_switches": ru.ru_nivcsw, "voluntary_switches": ru.ru_nvcsw}
+    for path, scale in (("/sys/fs/cgroup/cpu.stat", 1.0), ("/sys/fs/cgroup/cpu/cpu.stat", 1e-3)):
+        try:
+            kv = dict(line.split() for line in open(path) if len(line.split()) == 2)
+        except OSError:
+            continue
+        snap["nr_throttled"] = int(kv.get("nr_throttled", 0))
+        t = kv.get("throttled_usec", kv.get("throttled_time", 0))
+        snap["throttled_us"] = float(t) * scale
+        break
+    return snap
+
+
 def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads, inflight):
     """Poisson arrivals of the 5-set mix over the resident key pool; p50/p99 per rate."""
     rng = np.random.default_rng(seed)
@@ -304,6 +330,7 @@ def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads, inflight):
             arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
             kid_seq, ct_seq = t_kid[picks], t_ct[picks]
             srv.reset_stats()
+            sched0 = cpu_sched_snapshot()
             rids = np.zeros(n_req, np.uint64)
             start = time.perf_counter()
             i = 0
@@ -320,19 +347,40 @@ def serve_leg(cm, ctx, pool, rates, duration_s, seed, threads, inflight):
                         time.sleep(gap * 0.5)
             got = srv.wait_batch(rids, 1024, 60e6)
             elapsed = time.perf_counter() - start
+            sched1 = cpu_sched_snapshot()
             ok = bool(np.array_equal(got, t_e[picks]))
             st = srv.stats()
+            host = {k: round(sched1[k] - sched0[k], 1) for k in sched1 if k in sched0}
+            lg = srv.launch_log()
+            parts = {}
+            if lg.size:
+                # where a launch's time goes: host enqueue, GPU span, the rest of enqueue ->
+                # completion seen (stream start latency and polling), result delivery
+                for name, v in (("enqueue", lg["enqueue_us"]), ("gpu", lg["gpu_us"]),
+                                ("seen_minus_gpu", lg["seen_us"] - lg["gpu_us"]), ("retire", lg["retire_us"])):
+                    parts[name] = {"p50": round(float(np.percentile(v, 50)), 1),
+                                   "p99": round(float(np.percentile(v, 99)), 1)}
             out.append({"offered_req_per_s": rate, "requests": n_req, "dec_per_s": round(n_req / elapsed, 1),
                         "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
                         "p999_us": round(st["p999_us"], 1), "max_us": round(st["max_us"], 1),
                         "queue_p99_us": round(st["queue_p99_us"], 1), "queue_max_us": round(st["queue_max_us"], 1),
                         "launch_p99_us": round(st["device_p99_us"], 1), "launch_max_us": round(st["device_max_us"], 1),
+                        "launch_gpu_p99_us": round(st["gpu_p99_us"], 1), "launch_gpu_max_us": round(st["gpu_max_us"], 1),
                         "failed": st["failed"], "mean_batch": round(st["mean_batch"], 1),
-                        "launches": st["launches"], "bit_exact": ok})
+                        "launches": st["launches"], "bit_exact": ok,
+                        "worker_step_max_us": {k: round(st[k + "_max_us"], 1) for k in
+                                               ("step_launch", "step_retire", "step_poll")},
+                        "worker_wait_overshoot_max_us": round(st["wait_overshoot_max_us"], 1),
+                        "host_scheduling": host, "launch_parts_us": parts})
             log(f"serve rate {rate:.0f}/s: {n_req / elapsed:,.0f} dec/s p50 {st['p50_us']:.0f}us "
                 f"p99 {st['p99_us']:.0f}us p99.9 {st['p999_us']:.0f}us max {st['max_us']:.0f}us "
-                f"(queue max {st['queue_max_us']:.0f}us, launch max {st['device_max_us']:.0f}us) "
-                f"mean batch {st['mean_batch']:.1f} exact={ok}")
+                f"(queue p99/max {st['queue_p99_us']:.0f}/{st['queue_max_us']:.0f}us, launch p99/max "
+                f"{st['device_p99_us']:.0f}/{st['device_max_us']:.0f}us, of it on the GPU "
+                f"{st['gpu_p99_us']:.0f}/{st['gpu_max_us']:.0f}us) "
+                f"mean batch {st['mean_batch']:.1f} exact={ok}; worker steps max: launch "
+                f"{st['step_launch_max_us']:.0f}us retire {st['step_retire_max_us']:.0f}us poll "
+                f"{st['step_poll_max_us']:.0f}us, wait overshoot {st['wait_overshoot_max_us']:.0f}us; host {host}; "
+                f"launch parts p50/p99 " + ", ".join(f"{k} {v['p50']:.0f}/{v['p99']:.0f}" for k, v in parts.items()))
     finally:
         srv.close()
     return out
@@ -684,6 +732,7 @@ def main():
         "plain_6960119_4096": c2,
         "mixed_batch": mixed,
         "serve": serve,
+        "hip_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
         "key_admission_ms": key_load_ms,
         "keypair_generation": keygen,
         "cpu_baseline": cpu,

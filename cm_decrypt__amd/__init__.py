@@ -106,7 +106,10 @@ class ServerStats(_c.Structure):
                 ("launches", _c.c_uint64), ("failed", _c.c_uint64), ("mean_batch", _c.c_double),
                 ("p50_us", _c.c_double), ("p99_us", _c.c_double), ("p999_us", _c.c_double),
                 ("max_us", _c.c_double), ("queue_p99_us", _c.c_double), ("queue_max_us", _c.c_double),
-                ("device_p99_us", _c.c_double), ("device_max_us", _c.c_double), ("busy_s", _c.c_double)]
+                ("device_p99_us", _c.c_double), ("device_max_us", _c.c_double), ("gpu_p99_us", _c.c_double),
+                ("gpu_max_us", _c.c_double), ("busy_s", _c.c_double),
+                ("step_launch_max_us", _c.c_double), ("step_retire_max_us", _c.c_double),
+                ("step_poll_max_us", _c.c_double), ("wait_overshoot_max_us", _c.c_double)]
 
     def as_dict(self):
         return {n: (float(getattr(self, n)) if t is _c.c_double else int(getattr(self, n)))
@@ -174,6 +177,7 @@ _sig("cmd_server_poll", _I, _P, _U64, _P, _SZ)
 _sig("cmd_server_wait", _I, _P, _U64, _P, _SZ, _c.c_double)
 _sig("cmd_server_latencies", _I, _P, _P, _P, _SZ, _c.POINTER(_SZ))
 _sig("cmd_server_stats_get", _I, _P, _c.POINTER(ServerStats))
+_sig("cmd_server_launch_log", _I, _P, _P, _SZ, _c.POINTER(_SZ))
 _sig("cmd_server_reset_stats", _I, _P)
 _sig("cmd_server_destroy", _I, _P)
 _sig("cmd_set_kernel_timing", _I, _P, _I32)
@@ -661,6 +665,17 @@ class Server:
         _check(_lib.cmd_server_latencies(self._h, _ptr(tot), _ptr(que), n.value, _c.byref(n)),
                "cmd_server_latencies")
         return tot, que
+
+    LAUNCH_DTYPE = np.dtype([("t_us", "<f8"), ("enqueue_us", "<f8"), ("gpu_us", "<f8"), ("seen_us", "<f8"),
+                             ("retire_us", "<f8"), ("items", "<u4"), ("pad", "<u4")])
+
+    def launch_log(self) -> np.ndarray:
+        """Per-launch timing records since the last reset_stats() (cmd_server_launch)."""
+        n = _SZ(0)
+        _check(_lib.cmd_server_launch_log(self._h, None, 0, _c.byref(n)), "cmd_server_launch_log")
+        out = np.zeros(n.value, self.LAUNCH_DTYPE)
+        _check(_lib.cmd_server_launch_log(self._h, _ptr(out), n.value, _c.byref(n)), "cmd_server_launch_log")
+        return out[:n.value]
 
     def stats(self) -> dict:
         st = ServerStats()

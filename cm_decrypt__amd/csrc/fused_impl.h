@@ -592,16 +592,24 @@ struct FusedTileKernel {
     }
     static_assert(32 * 64 * R <= kWaveLds, "four-row tables must fit the wave's LDS");
 
-    // Bytes 4w .. 4w+3 of a ciphertext (those that exist) as a little-endian word.
-    static __device__ __forceinline__ uint32_t ct_word(const uint8_t *c, int w)
+    // Bytes 4w .. 4w+3 of a ciphertext (those that exist) as a little-endian word; one dword
+    // load when the rows are 4-byte aligned (al) and the word is whole.
+    static __device__ __forceinline__ uint32_t ct_word(const uint8_t *c, int w, bool al)
     {
         uint32_t v = 0;
         if (c) {
+            if (al && 4 * w + 4 <= Pm::synd_bytes) return *reinterpret_cast<const uint32_t *>(c + 4 * w);
 #pragma unroll
             for (int b = 0; b < 4; b++)
                 if (4 * w + b < Pm::synd_bytes) v |= (uint32_t)c[4 * w + b] << (8 * b);
         }
         return v;
+    }
+    static_assert(WORDS <= 64, "a ciphertext's words fit one per lane");
+
+    static __device__ __forceinline__ bool ct_aligned(const FusedArgs &a)
+    {
+        return ((a.ct_stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(a.ct) & 3u) == 0);
     }
 
     // ---- stage 1: syndromes into LDS bit planes ---------------------------------------------
@@ -611,6 +619,7 @@ struct FusedTileKernel {
     // ciphertext, only the rows of set bits.
     static __device__ __forceinline__ void syndrome(const FusedArgs &a, const FusedTile &tl, uint32_t *lds, int lane)
     {
+        const bool al = ct_aligned(a);
         const int chunks = (tl.count + CH - 1) / CH;
         for (int c = 0; c < chunks; c++) {
             const int x0 = c * CH;
@@ -652,8 +661,10 @@ struct FusedTileKernel {
                     gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
                     uint32_t va[8][R], vb[8][R], cw[CH];
                     load_group(rows, 0, lane, va);
+                    uint32_t next = ct_word(my_ct, 0, al);  // the next word's load is in flight
                     for (int w = 0; w < WORDS; w++) {
-                        const uint32_t mine = ct_word(my_ct, w);
+                        const uint32_t mine = next;
+                        if (w + 1 < WORDS) next = ct_word(my_ct, w + 1, al);
 #pragma unroll
                         for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
                         const int g = 4 * w;
@@ -679,12 +690,10 @@ struct FusedTileKernel {
                     const KeyDev *kd = &a.keys[a.item_key[tl.first + x0 + x]];
                     gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
                     const uint8_t *ctp = a.ct + (size_t)a.items[tl.first + x0 + x] * a.ct_stride;
+                    // the whole ciphertext in one load, word w in lane w
+                    const uint32_t ctw = lane < WORDS ? ct_word(ctp, lane, al) : 0u;
                     for (int w = 0; w < WORDS; w++) {
-                        uint32_t cwv = 0;
-#pragma unroll
-                        for (int b = 0; b < 4; b++)
-                            if (4 * w + b < Pm::synd_bytes) cwv |= (uint32_t)ctp[4 * w + b] << (8 * b);
-                        uint32_t bits = uni(cwv);
+                        uint32_t bits = (uint32_t)__builtin_amdgcn_readlane((int)ctw, w);
                         gu32 *wrows = rows + (size_t)(w * 32) * PAIRS;
                         while (bits) {
                             uint32_t v[GRP][R];
@@ -1294,12 +1303,14 @@ struct FusedTileKernel {
     // All row groups of one key's table for the ciphertexts of a run (others read as zero),
     // software-pipelined: the next group's loads are in flight while the current one is added.
     static __device__ __forceinline__ void stream_col(gu32 *rows, const uint8_t *my_ct, int col,
-                                                      uint32_t (&acc)[CH], uint32_t *tab, int lane)
+                                                      uint32_t (&acc)[CH], uint32_t *tab, int lane, bool al)
     {
         uint32_t va[8], vb[8], cw[CH];
         load_group_col(rows, 0, col, va);
+        uint32_t next = ct_word(my_ct, 0, al);  // the next word's load is in flight
         for (int w = 0; w < WORDS; w++) {
-            const uint32_t mine = ct_word(my_ct, w);
+            const uint32_t mine = next;
+            if (w + 1 < WORDS) next = ct_word(my_ct, w + 1, al);
 #pragma unroll
             for (int x = 0; x < CH; x++) cw[x] = (uint32_t)__builtin_amdgcn_readlane((int)mine, x);
             const int g = 4 * w;
@@ -1318,6 +1329,7 @@ struct FusedTileKernel {
     static __device__ __forceinline__ void syndrome_pair(const FusedArgs &a, const FusedTile &tl, uint32_t *lds,
                                                          int lane, int wv)
     {
+        const bool al = ct_aligned(a);
         const int col = lane + 64 * wv;                // this wave's table column
         const bool has_col = R == 2 || wv == 0;        // t <= 64: wave 0 owns every column
         const bool live = lane < CH && lane < tl.count;
@@ -1341,7 +1353,7 @@ struct FusedTileKernel {
                 rem &= ~__ballot(in_run);
                 const KeyDev *kd = &a.keys[key];
                 stream_col(gptr(reinterpret_cast<const uint32_t *>(kd->table)), in_run ? ct_ptr : nullptr, col, acc,
-                           tab, lane);
+                           tab, lane, al);
             }
         } else if (has_col) {
             // one key per ciphertext: only the rows of its set bits (the ciphertext is public),
@@ -1353,8 +1365,10 @@ struct FusedTileKernel {
                 const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
                 gu32 *rows = gptr(reinterpret_cast<const uint32_t *>(kd->table));
                 const uint8_t *ctp = a.ct + (size_t)a.items[tl.first + x] * a.ct_stride;
+                // the whole ciphertext in one load, word w in lane w
+                const uint32_t ctw = lane < WORDS ? ct_word(ctp, lane, al) : 0u;
                 for (int w = 0; w < WORDS; w++) {
-                    uint32_t bits = uni(ct_word(ctp, w));
+                    uint32_t bits = (uint32_t)__builtin_amdgcn_readlane((int)ctw, w);
                     gu32 *wrows = rows + (size_t)(w * 32) * PAIRS;
                     while (bits) {
                         uint32_t v[GRP];

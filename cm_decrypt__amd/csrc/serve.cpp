@@ -33,6 +33,8 @@
 #include <memory>
 #include <thread>
 
+#include <sys/prctl.h>
+
 #include "engine.h"
 
 using namespace cmd;
@@ -46,7 +48,8 @@ double now_us()
     return std::chrono::duration<double, std::micro>(Clock::now().time_since_epoch()).count();
 }
 
-enum ReqState : uint8_t { kFree = 0, kPending = 1, kInFlight = 2, kDone = 3 };
+// kTaken: finished and being copied out by cmd_server_wait_batch without the lock held
+enum ReqState : uint8_t { kFree = 0, kPending = 1, kInFlight = 2, kDone = 3, kTaken = 4 };
 
 struct Req {
     uint32_t gen = 0;
@@ -66,15 +69,16 @@ constexpr uint32_t kBlockSize = 1u << kBlockBits;
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t start = nullptr, done = nullptr;  // around the launch's device work (timing events)
     uint8_t *h_in = nullptr, *h_e = nullptr;  // pinned staging: [ciphertexts | work list], results
+    uint8_t *h_e_dev = nullptr;               // h_e as the device addresses it
     DevBuf dev;                               // the same input image, then d_e
     DevBuf kem;                               // decapsulation scratch (accept flags, SHAKE256 inputs)
     std::vector<uint32_t> reqs;               // pool indices of the launch, in item order
     size_t n_launch = 0;                      // leading reqs that are in the launch (the rest failed)
     bool busy = false;                        // reserved by the worker (being launched or in flight)
     bool launched = false;                    // its completion event is recorded
-    double t_launch = 0;
+    double t_launch = 0, t_enq = 0, t_seen = 0;  // popped, enqueued, completion seen (host clock)
 };
 
 }  // namespace
@@ -94,7 +98,10 @@ struct cmd_server {
     bool stop = false;
     // statistics
     uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_items = 0, failed = 0;
-    std::vector<double> lat_us, queue_us, dev_us;
+    std::vector<double> lat_us, queue_us, dev_us, gpu_us;
+    std::vector<cmd_server_launch> launch_log;
+    double t_created = 0;
+    double step_max[4] = {0, 0, 0, 0};  // launch, retire, poll, wait overshoot (us)
     double busy_since = -1, busy_us = 0;
     int busy_slots = 0;
     size_t in_bytes = 0;  // capacity of a launch's input image
@@ -238,6 +245,11 @@ int launch_slot(cmd_server *srv, Slot &s)
     std::memset(s.h_in + off_ctr, 0, 4);
     char *d_in = (char *)s.dev.ptr;
     uint8_t *d_e = (uint8_t *)(d_in + srv->in_bytes);
+    // A launch of decryptions only writes its error vectors straight into the pinned result
+    // buffer (no copy back after the kernel); decapsulations read e again on the device, so a
+    // launch with any keeps e in device memory and copies the results back.
+    const bool direct = kem.empty();
+    CMD_HIP(hipEventRecord(s.start, s.stream));
     CMD_HIP(hipMemcpyAsync(d_in, s.h_in, in_bytes, hipMemcpyHostToDevice, s.stream));
     FusedArgs a;
     a.tiles = (const FusedTile *)(d_in + off_tiles);
@@ -247,7 +259,7 @@ int launch_slot(cmd_server *srv, Slot &s)
     a.item_key = (const int32_t *)(d_in + off_keys);
     a.keys = ctx->d_keys;
     a.ct = (const uint8_t *)d_in;
-    a.e = d_e;
+    a.e = direct ? s.h_e_dev : d_e;
     a.ct_stride = kCtStride;
     a.e_stride = kEStride;
     a.tile_counter = (uint32_t *)(d_in + off_ctr);
@@ -291,7 +303,7 @@ int launch_slot(cmd_server *srv, Slot &s)
         h.key_stride = kEStride;
         CMD_HIP(launch_kem_hash(h, s.stream));
     }
-    CMD_HIP(hipMemcpyAsync(s.h_e, d_e, count * kEStride, hipMemcpyDeviceToHost, s.stream));
+    if (!direct) CMD_HIP(hipMemcpyAsync(s.h_e, d_e, count * kEStride, hipMemcpyDeviceToHost, s.stream));
     CMD_HIP(hipEventRecord(s.done, s.stream));
     return CMD_OK;
 }
@@ -303,6 +315,9 @@ void retire_slot(cmd_server *srv, Slot &s, std::unique_lock<std::mutex> &lk, int
 {
     lk.unlock();
     const double t = now_us();
+    float gpu_ms = -1.f;
+    if (launch_rc == CMD_OK && s.n_launch && hipEventElapsedTime(&gpu_ms, s.start, s.done) != hipSuccess)
+        gpu_ms = -1.f;
     for (size_t i = 0; i < s.reqs.size(); i++) {
         Req &r = srv->req(s.reqs[i]);
         if (r.rc == CMD_OK && launch_rc != CMD_OK) r.rc = launch_rc;
@@ -322,6 +337,18 @@ void retire_slot(cmd_server *srv, Slot &s, std::unique_lock<std::mutex> &lk, int
         if (r.rc != CMD_OK) srv->failed++;
     }
     if (launch_rc == CMD_OK && s.n_launch) srv->dev_us.push_back(t - s.t_launch);
+    if (gpu_ms >= 0.f) {
+        srv->gpu_us.push_back(1e3 * gpu_ms);
+        cmd_server_launch rec;
+        rec.t_us = s.t_launch - srv->t_created;
+        rec.enqueue_us = s.t_enq - s.t_launch;
+        rec.gpu_us = 1e3 * gpu_ms;
+        rec.seen_us = s.t_seen - s.t_enq;
+        rec.retire_us = now_us() - s.t_seen;
+        rec.items = (uint32_t)s.n_launch;
+        rec.pad = 0;
+        srv->launch_log.push_back(rec);
+    }
     s.reqs.clear();
     s.n_launch = 0;
     s.busy = false;
@@ -332,17 +359,32 @@ void retire_slot(cmd_server *srv, Slot &s, std::unique_lock<std::mutex> &lk, int
     }
 }
 
+// The longest worker step of each kind since the last reset.  mu held.
+void note_step(cmd_server *srv, int kind, double us)
+{
+    if (us > srv->step_max[kind]) srv->step_max[kind] = us;
+}
+
+constexpr int kPollUs = 10;  // completion polling interval while launches are in flight
+
 void worker_main(cmd_server *srv)
 {
     (void)hipSetDevice(srv->ctx->device);
+    // the completion polls sleep kPollUs; the default timer slack (50 us) would stretch each
+    prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
     std::unique_lock<std::mutex> lk(srv->mu);
     for (;;) {
         bool progressed = false;
         for (Slot &s : srv->slots) {
             if (!s.busy || !s.launched) continue;
+            const double tq = now_us();
             hipError_t q = hipEventQuery(s.done);
+            const double tr = now_us();
+            note_step(srv, 2, tr - tq);
             if (q == hipSuccess) {
+                s.t_seen = tr;
                 retire_slot(srv, s, lk, CMD_OK);
+                note_step(srv, 1, now_us() - tr);
                 progressed = true;
             } else if (q != hipErrorNotReady) {
                 const int rc = hip_fail(q, "server launch");
@@ -380,8 +422,10 @@ void worker_main(cmd_server *srv)
                 srv->launched_items += s.reqs.size();
                 lk.unlock();
                 int rc = launch_slot(srv, s);
+                s.t_enq = now_us();
                 if (rc) (void)hipStreamSynchronize(s.stream);  // copies already queued finish first
                 lk.lock();
+                note_step(srv, 0, now_us() - t);
                 if (rc || s.n_launch == 0) {
                     retire_slot(srv, s, lk, rc);
                     srv->cv_done.notify_all();
@@ -394,9 +438,11 @@ void worker_main(cmd_server *srv)
         if (srv->stop && srv->pending.empty() && srv->busy_slots == 0) break;
         if (!progressed) {
             // poll launches in flight often; otherwise sleep until a submit arrives
-            if (srv->busy_slots > 0 || !srv->pending.empty())
-                srv->cv_work.wait_for(lk, std::chrono::microseconds(10));
-            else
+            if (srv->busy_slots > 0 || !srv->pending.empty()) {
+                const double tw = now_us();
+                srv->cv_work.wait_for(lk, std::chrono::microseconds(kPollUs));
+                note_step(srv, 3, now_us() - tw - kPollUs);
+            } else
                 srv->cv_work.wait(lk);
         }
     }
@@ -486,7 +532,7 @@ int submit_impl(cmd_server *s, int32_t mode, size_t count, const int32_t *key_id
 int take_result(cmd_server *s, uint64_t id, uint8_t *out, size_t cap)
 {
     Req *r = s->lookup(id);
-    if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+    if (!r || r->state == kTaken) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)id);
     if (r->state != kDone) return CMD_ERR_PENDING;
     const int rc = r->rc;
     if (rc == CMD_OK && out) {
@@ -544,6 +590,9 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     srv->lat_us.reserve(1 << 20);
     srv->queue_us.reserve(1 << 20);
     srv->dev_us.reserve(1 << 16);
+    srv->gpu_us.reserve(1 << 16);
+    srv->launch_log.reserve(1 << 16);
+    srv->t_created = now_us();
     // a first few blocks up front; submit grows the pool ahead of need
     for (int b = 0; b < 4 && srv->pool_size < srv->pool_cap; b++) {
         std::unique_ptr<Req[]> blk = cmd_server::new_block();
@@ -554,10 +603,12 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     CMD_HIP(hipSetDevice(ctx->device));
     int rc = CMD_OK;
     for (Slot &s : srv->slots) {
+        // results: pinned, mapped and coherent, so the kernel's stores reach host memory
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreate(&s.start) != hipSuccess || hipEventCreate(&s.done) != hipSuccess ||
             hipHostMalloc((void **)&s.h_in, in_bytes, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_e, mb * kEStride, hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void **)&s.h_e, mb * kEStride, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&s.h_e_dev, s.h_e, 0) != hipSuccess) {
             rc = set_error(CMD_ERR_HIP, "server slot allocation failed");
             break;
         }
@@ -567,6 +618,7 @@ extern "C" int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd
     if (rc) {
         for (Slot &s : srv->slots) {
             if (s.stream) (void)hipStreamDestroy(s.stream);
+            if (s.start) (void)hipEventDestroy(s.start);
             if (s.done) (void)hipEventDestroy(s.done);
             if (s.h_in) (void)hipHostFree(s.h_in);
             if (s.h_e) (void)hipHostFree(s.h_e);
@@ -636,30 +688,67 @@ extern "C" int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t
                                      size_t out_stride, double timeout_us)
 {
     if (!s || (count && (!reqs || !out))) return CMD_ERR_ARG;
-    std::unique_lock<std::mutex> lk(s->mu);
     const auto deadline = Clock::now() + std::chrono::microseconds((long long)(timeout_us < 0 ? 0 : timeout_us));
-    for (size_t i = 0; i < count; i++) {
-        const Req *r = s->lookup(reqs[i]);
-        if (!r) return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)reqs[i]);
-        if ((size_t)r->out_bytes > out_stride)
-            return set_error(CMD_ERR_ARG, "out_stride %zu below the result of request %zu", out_stride, i);
+    // The lock is held for at most kChunk requests at a time: the worker needs it to retire
+    // launches, and a long hold here would stall every request in flight.
+    constexpr size_t kChunk = 256;
+    for (size_t i0 = 0; i0 < count; i0 += kChunk) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (size_t i = i0; i < std::min(count, i0 + kChunk); i++) {
+            const Req *r = s->lookup(reqs[i]);
+            if (!r || r->state == kTaken)
+                return set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)reqs[i]);
+            if ((size_t)r->out_bytes > out_stride)
+                return set_error(CMD_ERR_ARG, "out_stride %zu below the result of request %zu", out_stride, i);
+        }
     }
     int first_err = CMD_OK;
-    for (size_t i = 0; i < count; i++) {
-        for (;;) {
-            int rc = take_result(s, reqs[i], out + i * out_stride, out_stride);
-            if (rc == CMD_ERR_PENDING) {
+    std::vector<uint32_t> taken;
+    std::vector<size_t> pos;
+    taken.reserve(kChunk);
+    pos.reserve(kChunk);
+    std::unique_lock<std::mutex> lk(s->mu);
+    size_t i = 0;
+    while (i < count) {
+        // finished requests, in order, up to a chunk; wait only when nothing is taken yet
+        taken.clear();
+        pos.clear();
+        int stop_rc = CMD_OK;
+        while (i < count && taken.size() < kChunk) {
+            Req *r = s->lookup(reqs[i]);
+            if (!r || r->state == kTaken) {
+                stop_rc = set_error(CMD_ERR_ARG, "unknown request %llu", (unsigned long long)reqs[i]);
+                break;
+            }
+            if (r->state != kDone) {
+                if (!taken.empty()) break;
                 if (timeout_us < 0) {
                     s->cv_done.wait(lk);
-                    continue;
+                } else if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && r->state != kDone) {
+                    stop_rc = CMD_ERR_TIMEOUT;
+                    break;
                 }
-                if (s->cv_done.wait_until(lk, deadline) != std::cv_status::timeout) continue;
-                rc = take_result(s, reqs[i], out + i * out_stride, out_stride);
-                if (rc == CMD_ERR_PENDING) return CMD_ERR_TIMEOUT;
+                continue;
             }
-            if (rc != CMD_OK && first_err == CMD_OK) first_err = rc;
-            break;
+            r->state = kTaken;
+            taken.push_back((uint32_t)reqs[i]);
+            pos.push_back(i);
+            i++;
         }
+        lk.unlock();
+        for (size_t k = 0; k < taken.size(); k++) {
+            const Req &r = s->req(taken[k]);
+            if (r.rc == CMD_OK)
+                std::memcpy(out + pos[k] * out_stride, r.e, r.out_bytes);
+            else if (first_err == CMD_OK)
+                first_err = r.rc;
+        }
+        lk.lock();
+        for (uint32_t idx : taken) {
+            s->req(idx).state = kFree;
+            s->free_idx.push_back(idx);
+        }
+        if (stop_rc != CMD_OK) return stop_rc;
     }
     return first_err;
 }
@@ -667,7 +756,7 @@ extern "C" int cmd_server_wait_batch(cmd_server *s, size_t count, const uint64_t
 extern "C" int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out)
 {
     if (!s || !out) return CMD_ERR_ARG;
-    std::vector<double> lat, queue, dev;
+    std::vector<double> lat, queue, dev, gpu;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         out->submitted = s->submitted;
@@ -679,9 +768,14 @@ extern "C" int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out)
         double busy = s->busy_us;
         if (s->busy_since >= 0) busy += now_us() - s->busy_since;
         out->busy_s = busy * 1e-6;
+        out->step_launch_max_us = s->step_max[0];
+        out->step_retire_max_us = s->step_max[1];
+        out->step_poll_max_us = s->step_max[2];
+        out->wait_overshoot_max_us = s->step_max[3];
         lat = s->lat_us;
         queue = s->queue_us;
         dev = s->dev_us;
+        gpu = s->gpu_us;
     }
     out->p50_us = percentile(lat, 0.50);
     out->p99_us = percentile(lat, 0.99);
@@ -691,6 +785,8 @@ extern "C" int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out)
     out->queue_max_us = percentile(queue, 1.0);
     out->device_p99_us = percentile(dev, 0.99);
     out->device_max_us = percentile(dev, 1.0);
+    out->gpu_p99_us = percentile(gpu, 0.99);
+    out->gpu_max_us = percentile(gpu, 1.0);
     return CMD_OK;
 }
 
@@ -707,6 +803,16 @@ extern "C" int cmd_server_latencies(cmd_server *s, double *total_us, double *que
     return CMD_OK;
 }
 
+extern "C" int cmd_server_launch_log(cmd_server *s, cmd_server_launch *out, size_t cap, size_t *n)
+{
+    if (!s || (cap && !out)) return CMD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const size_t k = std::min(cap, s->launch_log.size());
+    if (k) std::memcpy(out, s->launch_log.data(), k * sizeof(cmd_server_launch));
+    if (n) *n = s->launch_log.size();
+    return CMD_OK;
+}
+
 extern "C" int cmd_server_reset_stats(cmd_server *s)
 {
     if (!s) return CMD_ERR_ARG;
@@ -715,6 +821,9 @@ extern "C" int cmd_server_reset_stats(cmd_server *s)
     s->lat_us.clear();
     s->queue_us.clear();
     s->dev_us.clear();
+    s->gpu_us.clear();
+    s->launch_log.clear();
+    for (double &m : s->step_max) m = 0;
     s->busy_us = 0;
     if (s->busy_since >= 0) s->busy_since = now_us();
     return CMD_OK;
@@ -734,6 +843,7 @@ extern "C" int cmd_server_destroy(cmd_server *s)
     for (Slot &sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.start) (void)hipEventDestroy(sl.start);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.h_in) (void)hipHostFree(sl.h_in);
         if (sl.h_e) (void)hipHostFree(sl.h_e);

@@ -295,7 +295,14 @@ typedef struct {
     double   p50_us, p99_us, p999_us, max_us;  /* submit -> result available */
     double   queue_p99_us, queue_max_us;       /* submit -> launch enqueued */
     double   device_p99_us, device_max_us;     /* launch enqueued -> completion seen (per launch) */
+    double   gpu_p99_us, gpu_max_us;           /* input copy start -> results in host memory, on
+                                                  the launch stream (HIP events, per launch) */
     double   busy_s;                   /* time with at least one launch in flight */
+    /* Longest single step of the worker thread, by kind (wall time, microseconds): planning
+     * and enqueueing a launch, retiring one (result copies, bookkeeping), one pass of
+     * completion queries, and the overshoot of its timed wait for work beyond the time asked
+     * for.  A long wait overshoot with short steps means the thread was not scheduled. */
+    double   step_launch_max_us, step_retire_max_us, step_poll_max_us, wait_overshoot_max_us;
 } cmd_server_stats;
 
 int cmd_server_create(cmd_ctx *ctx, const cmd_server_config *cfg, cmd_server **out);
@@ -332,6 +339,17 @@ int cmd_server_stats_get(cmd_server *s, cmd_server_stats *out);
  * order: total (submit -> result) and queue (submit -> launch) microseconds.  Copies at most
  * cap of each; *n receives the number available. */
 int cmd_server_latencies(cmd_server *s, double *total_us, double *queue_us, size_t cap, size_t *n);
+/* One record per launch retired since the last reset (those whose device work completed),
+ * in retirement order: where a launch's time went, on the host clock except gpu_us. */
+typedef struct {
+    double   t_us;        /* requests taken from the queue, microseconds since server creation */
+    double   enqueue_us;  /* planning and enqueueing the copy and kernels (worker thread) */
+    double   gpu_us;      /* on the launch stream: input copy start -> results in host memory */
+    double   seen_us;     /* enqueue done -> completion seen by the worker (contains gpu_us) */
+    double   retire_us;   /* completion seen -> results delivered to the requests */
+    uint32_t items, pad;
+} cmd_server_launch;
+int cmd_server_launch_log(cmd_server *s, cmd_server_launch *out, size_t cap, size_t *n);
 int cmd_server_reset_stats(cmd_server *s);
 /* Stops admission, drains in-flight work, frees everything. */
 int cmd_server_destroy(cmd_server *s);
