@@ -524,6 +524,37 @@ def key_independence(cm, torch, ctx, sks, ct, key_of):
     return res
 
 
+STAGES = ((1, "syndrome"), (2, "berlekamp_massey"), (3, "fft_coefficient_phase"),
+          (0, "butterflies_benes_stores"))
+
+
+def stage_split(steps, warmup, schedules):
+    """Per-stage time of the headline launch: the launch timed with the stage-truncated
+    libraries (make -C cm_decrypt__amd stages: each tile stops after the syndrome, after
+    Berlekamp-Massey, after the FFT coefficient phase) and with the library itself, each in a
+    child process of its own; a stage's time is the difference of consecutive builds."""
+    import subprocess
+    here = os.path.join(ROOT, "cm_decrypt__amd")
+    total = {}
+    for stop, _ in STAGES:
+        lib = os.path.join(here, f"libcmdecrypt_stage{stop}.so" if stop else "libcmdecrypt.so")
+        if not os.path.exists(lib):
+            raise SystemExit(f"{lib} is missing: build it with `make -C {here} stages`")
+        env = dict(os.environ, CMD_DECRYPT_LIB=lib)
+        log(f"stage split: launch of {os.path.basename(lib)}")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--fused-only", "--steps", str(steps),
+                            "--warmup", str(warmup), "--schedules", schedules],
+                           env=env, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise SystemExit(f"stage build {stop} failed ({r.returncode}):\n{r.stderr[-2000:]}")
+        total[stop] = json.loads(r.stdout.strip().splitlines()[-1])["kernel_ms"]
+    prev, out = 0.0, {}
+    for stop, name in STAGES:
+        out[name] = round(total[stop] - prev, 4)
+        prev = total[stop]
+    return {"launch_ms": {str(k): v for k, v in total.items()}, "stage_ms": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -556,6 +587,10 @@ def main():
     ap.add_argument("--key-independence", action="store_true",
                     help="four fused launches of the same ciphertexts under two key assignments "
                          "(profile them with rocprofv3 --pmc) and exit")
+    ap.add_argument("--stage-split", action="store_true",
+                    help="time the headline launch with the stage-truncated libraries and exit")
+    ap.add_argument("--fused-only", action="store_true",
+                    help="time only the headline fused launch (used by --stage-split) and exit")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -567,6 +602,11 @@ def main():
                                    "of the keys",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.stage_split:
+        print(json.dumps({"stage_split": stage_split(args.steps, args.warmup, args.schedules),
+                          "workload": "mceliece8192128 x 16 keys x 65536 ciphertexts"}), flush=True)
+        return
 
     import torch
     import cm_decrypt__amd as cm
@@ -621,6 +661,11 @@ def main():
     log(f"fused: {value:,.0f} dec/s, {wall / args.steps * 1e3:.3f} ms/step, kernel {kern_ms:.3f} ms")
     e_fused = d_e.cpu().numpy()
     exact_fused = bool(np.array_equal(e_fused, e_true))
+    if args.fused_only:
+        print(json.dumps({"kernel_ms": round(kern_ms, 4), "ms_per_step": round(wall / args.steps * 1e3, 4),
+                          "bit_exact": exact_fused, "schedule": sched,
+                          "lib": os.path.basename(cm.LIB_PATH)}), flush=True)
+        return
 
     kem = None
     if not args.no_kem:

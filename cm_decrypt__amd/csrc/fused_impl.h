@@ -42,6 +42,14 @@
 
 namespace cmd {
 
+// Stage-truncated builds for timing the stages of the single-wavefront kernel (Makefile target
+// `stages`): 1 stops each tile after the syndrome, 2 after Berlekamp-Massey, 3 after the FFT
+// coefficient phase; 0 (the library) runs the whole decryption.  Truncated builds write no
+// error vectors.
+#ifndef CMD_STAGE_STOP
+#define CMD_STAGE_STOP 0
+#endif
+
 constexpr int kRowWords = 16;                          // LDS words per syndrome plane row (m used)
 constexpr int kWaveLds = (2 * kMaxT + 1) * kRowWords;  // words of LDS per wavefront
 
@@ -67,6 +75,58 @@ __device__ __forceinline__ uint32_t wave_xor_all(uint32_t v)
     v ^= dpp0<0x128>(v);  // row_ror:8
     return (uint32_t)(__builtin_amdgcn_readlane((int)v, 0) ^ __builtin_amdgcn_readlane((int)v, 16) ^
                       __builtin_amdgcn_readlane((int)v, 32) ^ __builtin_amdgcn_readlane((int)v, 48));
+}
+
+// XOR over the 64 lanes of each of N <= 16 values at once.  Each step folds one lane bit of
+// two registers into one: the permlane swaps exchange the halves (lane bit 5) or the odd and
+// even 16-lane rows (bit 4) of a register pair, so one XOR leaves the first value's partial
+// sums where that bit is 0 and the second's where it is 1; bits 2 and 3 do the same with a
+// lane-dependent select and one row rotation.  After the two in-quad steps every lane of quad
+// wave_xor_lane(k) / 4 holds the sum of value k.  `extra` is XORed into that register before
+// the in-quad steps (a term that a value's quad should add once).  About 3.5 instructions per
+// value against 11 for separate reductions.
+__device__ __forceinline__ constexpr int wave_xor_lane(int k)
+{
+    return ((k & 1) << 5) | (((k >> 1) & 1) << 4) | (((k >> 3) & 1) << 3) | (((k >> 2) & 1) << 2);
+}
+
+template <int N>
+__device__ __forceinline__ void wave_xor_many(const uint32_t (&v)[N], uint32_t extra, bool lane_b2, bool lane_b3,
+                                              uint32_t (&out)[N])
+{
+    static_assert(N >= 1 && N <= 16, "up to 16 values");
+    constexpr int N5 = (N + 1) / 2, N4 = (N5 + 1) / 2, N2 = (N4 + 1) / 2;
+    uint32_t r5[N5], r4[N4], r2[N2];
+#pragma unroll
+    for (int i = 0; i < N5; i++) {
+        const uint32_t a = v[2 * i], b = 2 * i + 1 < N ? v[2 * i + 1] : v[2 * i];
+        const auto s = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        r5[i] = s[0] ^ s[1];
+    }
+#pragma unroll
+    for (int i = 0; i < N4; i++) {
+        const uint32_t a = r5[2 * i], b = 2 * i + 1 < N5 ? r5[2 * i + 1] : r5[2 * i];
+        const auto s = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        r4[i] = s[0] ^ s[1];
+    }
+#pragma unroll
+    for (int i = 0; i < N2; i++) {
+        // lanes with bit 2 clear keep a and pair with a lane 4 away (bit 2 set) that sends its a
+        const uint32_t a = r4[2 * i], b = 2 * i + 1 < N4 ? r4[2 * i + 1] : r4[2 * i];
+        const uint32_t keep = lane_b2 ? b : a, send = lane_b2 ? a : b;
+        r2[i] = keep ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x124 /* row_ror:4 */, 0xF, 0xF, true);
+    }
+    uint32_t r;
+    {
+        const uint32_t a = r2[0], b = N2 > 1 ? r2[N2 - 1] : r2[0];
+        const uint32_t keep = lane_b3 ? b : a, send = lane_b3 ? a : b;
+        r = keep ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x128 /* row_ror:8 */, 0xF, 0xF, true);
+    }
+    r ^= extra;
+    r ^= dpp0<0xB1>(r);  // quad_perm [1,0,3,2]
+    r ^= dpp0<0x4E>(r);  // quad_perm [2,3,0,1]
+#pragma unroll
+    for (int k = 0; k < N; k++) out[k] = (uint32_t)__builtin_amdgcn_readlane((int)r, wave_xor_lane(k));
 }
 
 // Lane l receives v from lane l-1; lane 0 receives `carry`.
@@ -759,6 +819,13 @@ struct FusedTileKernel {
         const uint32_t top = (uint32_t)__builtin_amdgcn_readlane((int)v, M - 1);
         return and_xor(shift_up1_zero(v), pm, top);
     }
+    // The same with plane M-1 broadcast by DPP (row_newbcast: lane M-1 of each 16-lane row to
+    // its row; pm is zero outside row 0), so the value never leaves the vector registers.
+    static __device__ __forceinline__ uint32_t tx_mulx_v(uint32_t v, uint32_t pm)
+    {
+        const uint32_t top = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + (M - 1), 0xF, 0xF, true);
+        return and_xor(shift_up1_zero(v), pm, top);
+    }
 
     // Iteration n of the bit-sliced recurrence.  FULL: every lane block takes part and the
     // transposed top coefficient is live (n >= NFULL, a public bound), so the step is one
@@ -781,10 +848,30 @@ struct FusedTileKernel {
         return 64 * b <= LO + 2 ? true : (64 * b > HI + 1 ? false : 64 * b <= n + 2);
     }
 
+    // Bit-sliced D = n - 2L (two's complement, DBITS planes): the reference's test 2L <= n is
+    // D >= 0, and its length change L' = n + 1 - L turns D into -D - 1 = ~D, while an unchanged
+    // L makes it D + 1; so one sign plane and a select per plane replace the comparison and the
+    // subtraction on L.  |D| <= 2t + 2 < 2^(DBITS-1).
+    static constexpr int DBITS = 10;
+    static_assert(2 * T + 2 < (1 << (DBITS - 1)), "D fits its planes");
+
+    // Per-lane constants of the recurrence, computed once per tile.
+    struct BmLanes {
+        uint32_t keep_last;  // coefficients above T (last block) are dropped
+        uint32_t pm;         // planes of low(x) in lanes 0..M-1 (transposed top coefficient)
+        uint32_t top_addr;   // ds_bpermute source of plane k of the top term: lane k ...
+        uint32_t top_keep;   // ... for the lane wave_xor_lane(k) that reduces plane k
+        bool b2, b3;         // lane bits 2 and 3 (reduction selects)
+    };
+
+    // Iteration n of the bit-sliced recurrence.  The wave-uniform per-ciphertext state (b, D
+    // and the selection mask) stays in vector registers, where a select is one three-input
+    // operation instead of three scalar ones; the discrepancy planes come out of the reduction
+    // in scalar registers.
     template <int LO, int HI>
     static __device__ __forceinline__ void bm_step(int n, const uint4 *rows4, int lane, uint32_t (&C)[NBL][M],
-                                                   uint32_t (&B)[NBV][M], uint32_t (&bb)[M], uint32_t (&Lp)[9],
-                                                   uint32_t &ct, uint32_t &bt, uint32_t keep_last, uint32_t pm)
+                                                   uint32_t (&B)[NBV][M], uint32_t (&bb)[M], uint32_t (&Dp)[DBITS],
+                                                   uint32_t &ct, uint32_t &bt, const BmLanes &ln)
     {
         constexpr bool FULL = LO >= NFULL;
         // discrepancy d = sum_i C_i s_{n-i} (rows hold s_{j} at j+1, row 0 is zero)
@@ -805,8 +892,9 @@ struct FusedTileKernel {
             bs_mac_wide<M>(C[b], sv, p);
         }
         // + C_t s_{n-t} (TOPX): s_{n-t} (row n-t+1; row 0, zero, while n < t) as wave-uniform
-        // planes, Horner over them
-        [[maybe_unused]] uint32_t top = 0;
+        // planes, Horner over them; plane k of the result is in lane k, and goes to the lane
+        // that reduces plane k of the discrepancy
+        uint32_t extra = 0;
         if constexpr (TOPX && FULL) {
             const int row = n - T + 1 > 0 ? n - T + 1 : 0;
             uint32_t sv[16];
@@ -815,29 +903,25 @@ struct FusedTileKernel {
                 const uint4 w = rows4[row * 4 + q];
                 sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
             }
-            top = and_xor(0u, ct, uni(sv[M - 1]));
+            uint32_t top = and_xor(0u, ct, sv[M - 1]);
 #pragma unroll
-            for (int a = M - 2; a >= 0; a--) top = and_xor(tx_mulx(top, pm), ct, uni(sv[a]));
+            for (int a = M - 2; a >= 0; a--) top = and_xor(tx_mulx_v(top, ln.pm), ct, sv[a]);
+            extra = (uint32_t)__builtin_amdgcn_ds_bpermute((int)ln.top_addr, (int)top) & ln.top_keep;
         }
         uint32_t part[M], d[M];
         bs_fold<M>(p, part);
-#pragma unroll
-        for (int k = 0; k < M; k++) d[k] = wave_xor_all(part[k]);
-        if constexpr (TOPX && FULL) {
-#pragma unroll
-            for (int k = 0; k < M; k++) d[k] ^= (uint32_t)__builtin_amdgcn_readlane((int)top, k);
-        }
+        wave_xor_many<M>(part, extra, ln.b2, ln.b3, d);
         uint32_t mne = 0;
 #pragma unroll
         for (int k = 0; k < M; k++) mne |= d[k];
-        const uint32_t mle = mne & bs_le_const(Lp, (uint32_t)n >> 1);
+        const uint32_t mle = __builtin_amdgcn_bitop3_b32(mne, Dp[DBITS - 1], 0u, 0x30);  // mne & (D >= 0)
 
         // C' = b*C + d*B ; B' = mle ? C : B ; then B <- x*B
         if constexpr (TOPX && FULL) {
             // C_t' = b C_t + d B_t (B_t' comes from the shift below)
             uint32_t acc = and_xor(and_xor(0u, ct, bb[M - 1]), bt, d[M - 1]);
 #pragma unroll
-            for (int a = M - 2; a >= 0; a--) acc = and_xor(and_xor(tx_mulx(acc, pm), ct, bb[a]), bt, d[a]);
+            for (int a = M - 2; a >= 0; a--) acc = and_xor(and_xor(tx_mulx_v(acc, ln.pm), ct, bb[a]), bt, d[a]);
             ct = acc;
         }
 #pragma unroll
@@ -861,7 +945,8 @@ struct FusedTileKernel {
             }
         }
         // shift the highest block first so each carry is read before its source moves; B has
-        // degree <= n+1 here, so block b can receive a carry once 64b <= n+2
+        // degree <= n+1 here, so block b can receive a carry once 64b <= n+2.  The carry (lane
+        // 63 of the block below) reaches lane 0 by a wave rotation.
 #pragma unroll
         for (int b = NBV - 1; b >= 0; b--) {
             if (!shift_on<LO, HI>(b, n)) continue;
@@ -870,22 +955,28 @@ struct FusedTileKernel {
                 if (b == 0) {
                     B[b][k] = shift_up1_zero(B[b][k]);
                 } else {
-                    const uint32_t carry = (uint32_t)__builtin_amdgcn_readlane((int)B[b > 0 ? b - 1 : 0][k], 63);
-                    B[b][k] = shift_up1(B[b][k], carry);
+                    const uint32_t carry = (uint32_t)__builtin_amdgcn_update_dpp(
+                        0, (int)B[b > 0 ? b - 1 : 0][k], 0x13C /* wave_ror:1 */, 0xF, 0xF, true);
+                    B[b][k] = (uint32_t)__builtin_amdgcn_update_dpp((int)carry, (int)B[b][k], 0x138 /* wave_shr:1 */,
+                                                                    0xF, 0xF, false);
                 }
             }
         }
         if constexpr (!TOPX) {
 #pragma unroll
-            for (int k = 0; k < M; k++) B[NBV - 1][k] &= keep_last;
+            for (int k = 0; k < M; k++) B[NBV - 1][k] &= ln.keep_last;
         }
-        // L, b
-        uint32_t Ln[9];
-        bs_const_sub((uint32_t)n + 1, Lp, Ln);
+        // D' = mle ? ~D : D + 1 (plane 0 flips either way); b' = mle ? d : b
+        uint32_t cy = Dp[0];
+        Dp[0] = ~Dp[0];
 #pragma unroll
-        for (int q = 0; q < 9; q++) Lp[q] = (Ln[q] & mle) | (Lp[q] & ~mle);
+        for (int q = 1; q < DBITS; q++) {
+            const uint32_t ncy = __builtin_amdgcn_bitop3_b32(Dp[q], cy, 0u, 0xC0);  // D_q & carry
+            Dp[q] = __builtin_amdgcn_bitop3_b32(mle, Dp[q], cy, 0x36);              // mle ? ~D_q : D_q ^ carry
+            cy = ncy;
+        }
 #pragma unroll
-        for (int k = 0; k < M; k++) bb[k] = (d[k] & mle) | (bb[k] & ~mle);
+        for (int k = 0; k < M; k++) bb[k] = __builtin_amdgcn_bitop3_b32(mle, d[k], bb[k], 0xCA);
     }
 
     static __device__ __forceinline__ void berlekamp_massey(uint32_t *lds, int lane, uint32_t (&C)[NBL][M])
@@ -898,15 +989,25 @@ struct FusedTileKernel {
                 C[b][k] = (b == 0 && k == 0 && lane == 0) ? ~0u : 0u;  // C = 1
                 B[b][k] = (b == 0 && k == 0 && lane == 1) ? ~0u : 0u;  // B = x
             }
-        uint32_t bb[M], Lp[9];
+        uint32_t bb[M], Dp[DBITS];
 #pragma unroll
-        for (int k = 0; k < M; k++) bb[k] = k == 0 ? ~0u : 0u;
+        for (int k = 0; k < M; k++) bb[k] = k == 0 ? ~0u : 0u;  // b = 1
 #pragma unroll
-        for (int q = 0; q < 9; q++) Lp[q] = 0;
+        for (int q = 0; q < DBITS; q++) Dp[q] = 0;  // D = 0
 
-        // coefficients above T (last block) are dropped like the reference's fixed t+1 array
-        const uint32_t keep_last = (lane + 64 * (NBV - 1) <= T) ? ~0u : 0u;
-        const uint32_t pm = (lane < M && ((Modulus<M>::low >> lane) & 1u)) ? ~0u : 0u;
+        BmLanes ln;
+        ln.keep_last = (lane + 64 * (NBV - 1) <= T) ? ~0u : 0u;
+        ln.pm = (lane < M && ((Modulus<M>::low >> lane) & 1u)) ? ~0u : 0u;
+        ln.top_addr = 0;
+        ln.top_keep = 0;
+#pragma unroll
+        for (int k = 0; k < M; k++)
+            if (lane == wave_xor_lane(k)) {
+                ln.top_addr = 4u * k;
+                ln.top_keep = ~0u;
+            }
+        ln.b2 = (lane >> 2) & 1;
+        ln.b3 = (lane >> 3) & 1;
         uint32_t ct = 0, bt = 0;  // TOPX: planes of C_t and B_t, transposed
         const uint4 *rows4 = reinterpret_cast<const uint4 *>(lds);
 
@@ -916,13 +1017,13 @@ struct FusedTileKernel {
         constexpr int P1 = NBV > 1 ? 62 : NFULL, P2 = NBV > 1 ? 63 : NFULL;
         static_assert(P1 <= P2 && P2 <= NFULL, "phases in order");
 #pragma unroll 1
-        for (int n = 0; n < P1; n++) bm_step<0, P1>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+        for (int n = 0; n < P1; n++) bm_step<0, P1>(n, rows4, lane, C, B, bb, Dp, ct, bt, ln);
 #pragma unroll 1
-        for (int n = P1; n < P2; n++) bm_step<P1, P2>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+        for (int n = P1; n < P2; n++) bm_step<P1, P2>(n, rows4, lane, C, B, bb, Dp, ct, bt, ln);
 #pragma unroll 1
-        for (int n = P2; n < NFULL; n++) bm_step<P2, NFULL>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+        for (int n = P2; n < NFULL; n++) bm_step<P2, NFULL>(n, rows4, lane, C, B, bb, Dp, ct, bt, ln);
 #pragma unroll 1
-        for (int n = NFULL; n < 2 * T; n++) bm_step<NFULL, 2 * T>(n, rows4, lane, C, B, bb, Lp, ct, bt, keep_last, pm);
+        for (int n = NFULL; n < 2 * T; n++) bm_step<NFULL, 2 * T>(n, rows4, lane, C, B, bb, Dp, ct, bt, ln);
         if constexpr (TOPX) {
             // C_t into lane 0 of the last block
 #pragma unroll
@@ -1233,17 +1334,25 @@ struct FusedTileKernel {
     {
         syndrome(a, tl, lds, lane);
         wave_sync();
+        if constexpr (CMD_STAGE_STOP == 1) return;
         uint32_t C[NBL][M];
         if (tl.count <= SCALAR_BM)
             berlekamp_massey_scalar(lds, lane, tl.count, C);
         else
             berlekamp_massey(lds, lane, C);
+        if constexpr (CMD_STAGE_STOP == 2) {
+            // the locator's planes to LDS, so that the recurrence is not optimised away
+            write_coef_planes(C, lds, lane);
+            wave_sync();
+            return;
+        }
         if constexpr (ALG == 0) {
             write_coef_rows(C, lds, lane);
             roots(a, tl, lds, lane, e_aligned);
         } else {
             const FftTab *tab = a.fft + fft_variant(M, LOG);
             fft_coeff_phase(C, lds, lane, tab);
+            if constexpr (CMD_STAGE_STOP == 3) return;
             roots_fft(a, tl, lds, lane, e_aligned, tab);
         }
         wave_sync();
