@@ -414,20 +414,42 @@ struct FftPlan {
 
     // Exchanges lane bit LB with register bit Q: lane l keeps the words whose register bit Q
     // equals its lane bit LB and trades the others with lane l ^ 2^LB.
+    // Lane bits 5 and 4 are one permlane swap per word pair (the swap exchanges exactly the
+    // halves / 16-lane rows that change owner); bits 3 and 2 two row rotations that write
+    // only the 4-lane banks that receive; bits 1 and 0 a quad permutation and selects.
     template <int LB, int Q>
     static __device__ __forceinline__ void swap_bits(uint32_t (&W)[WPL][M], int lane)
     {
-        const bool upper = (opaque(lane) >> LB) & 1;
+        [[maybe_unused]] const bool upper = (opaque(lane) >> LB) & 1;
 #pragma unroll
         for (int r0 = 0; r0 < WPL; r0++) {
             if ((r0 >> Q) & 1) continue;
             const int r1 = r0 | (1 << Q);
 #pragma unroll
             for (int k = 0; k < M; k++) {
-                const uint32_t send = upper ? W[r0][k] : W[r1][k];
-                const uint32_t recv = xor_lanes<(1 << LB)>(send);
-                W[r0][k] = upper ? recv : W[r0][k];
-                W[r1][k] = upper ? W[r1][k] : recv;
+                const uint32_t a = W[r0][k], b = W[r1][k];
+                if constexpr (LB == 5) {
+                    const auto s = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+                    W[r0][k] = s[0];
+                    W[r1][k] = s[1];
+                } else if constexpr (LB == 4) {
+                    const auto s = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+                    W[r0][k] = s[0];
+                    W[r1][k] = s[1];
+                } else if constexpr (LB == 3) {
+                    // banks 2, 3 take b from 8 lanes down; banks 0, 1 take a from 8 lanes up
+                    W[r0][k] = (uint32_t)__builtin_amdgcn_update_dpp((int)a, (int)b, 0x128 /* row_ror:8 */, 0xF, 0xC, false);
+                    W[r1][k] = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)a, 0x128 /* row_ror:8 */, 0xF, 0x3, false);
+                } else if constexpr (LB == 2) {
+                    // banks 1, 3 take b from 4 lanes down; banks 0, 2 take a from 4 lanes up
+                    W[r0][k] = (uint32_t)__builtin_amdgcn_update_dpp((int)a, (int)b, 0x124 /* row_ror:4 */, 0xF, 0xA, false);
+                    W[r1][k] = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)a, 0x12C /* row_ror:12 */, 0xF, 0x5, false);
+                } else {
+                    const uint32_t send = upper ? a : b;
+                    const uint32_t recv = xor_lanes<(1 << LB)>(send);
+                    W[r0][k] = upper ? recv : a;
+                    W[r1][k] = upper ? b : recv;
+                }
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -639,12 +661,24 @@ struct FusedTileKernel {
                 tab_put(tab, 16 * h + (i ^ (i >> 1)), lane, cur);
             }
         }
+        // the entries of ciphertext x + PD are read while x adds its own, so PD lookups are in
+        // flight instead of one LDS round trip per ciphertext
+        constexpr int PD = 4;
+        uint2 lo[PD], hi[PD];
+#pragma unroll
+        for (int x = 0; x < PD; x++) {
+            lo[x] = tab_get(tab, (cw[x] >> BIT0) & 15u, lane);
+            hi[x] = tab_get(tab, 16u + ((cw[x] >> (BIT0 + 4)) & 15u), lane);
+        }
 #pragma unroll
         for (int x = 0; x < CH; x++) {
-            const uint2 lo = tab_get(tab, (cw[x] >> BIT0) & 15u, lane);
-            const uint2 hi = tab_get(tab, 16u + ((cw[x] >> (BIT0 + 4)) & 15u), lane);
-            acc[x][0] = xor3(acc[x][0], lo.x, hi.x);
-            if constexpr (R == 2) acc[x][R - 1] = xor3(acc[x][R - 1], lo.y, hi.y);
+            const uint2 l = lo[x % PD], h = hi[x % PD];
+            if (x + PD < CH) {
+                lo[x % PD] = tab_get(tab, (cw[x + PD] >> BIT0) & 15u, lane);
+                hi[x % PD] = tab_get(tab, 16u + ((cw[x + PD] >> (BIT0 + 4)) & 15u), lane);
+            }
+            acc[x][0] = xor3(acc[x][0], l.x, h.x);
+            if constexpr (R == 2) acc[x][R - 1] = xor3(acc[x][R - 1], l.y, h.y);
             // pin the sum here: left free, the compiler defers these XORs and keeps the
             // loaded entries of many groups live, which overflows the register file
             asm volatile("" : "+v"(acc[x][0]), "+v"(acc[x][R - 1]));
