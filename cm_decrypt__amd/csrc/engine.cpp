@@ -1,5 +1,6 @@
 // Host engine behind the C ABI: contexts, resident keys, batch planning and launch.
 #include "engine.h"
+#include "trace.h"
 
 #include <algorithm>
 #include <thread>
@@ -319,10 +320,19 @@ int batch_enqueue(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
         for (const auto &g : b->groups) {
             const KeyDev &key = ctx->keys[g.key].dev;
             const int32_t *idx = b->d_idx + g.first;
-            CMD_HIP(launch_plain_synd(g.pidx, key, g.count, idx, ct, b->ct_stride, (uint16_t *)b->d_s.ptr, st));
-            CMD_HIP(launch_plain_bm(g.pidx, g.count, (const uint16_t *)b->d_s.ptr, (uint16_t *)b->d_loc.ptr, st));
-            CMD_HIP(launch_plain_chien(g.pidx, key, g.count, idx, (const uint16_t *)b->d_loc.ptr, e, b->e_stride,
-                                       st));
+            {
+                TraceRange r("synd");
+                CMD_HIP(launch_plain_synd(g.pidx, key, g.count, idx, ct, b->ct_stride, (uint16_t *)b->d_s.ptr, st));
+            }
+            {
+                TraceRange r("bm");
+                CMD_HIP(launch_plain_bm(g.pidx, g.count, (const uint16_t *)b->d_s.ptr, (uint16_t *)b->d_loc.ptr, st));
+            }
+            {
+                TraceRange r("chien");
+                CMD_HIP(launch_plain_chien(g.pidx, key, g.count, idx, (const uint16_t *)b->d_loc.ptr, e, b->e_stride,
+                                           st));
+            }
         }
         return CMD_OK;
     }
@@ -356,9 +366,13 @@ int enqueue_fused(cmd_ctx *ctx, int mode, const FusedSchedule sched[], const std
     a.tile_counter = d_counter;
     a.fft = ctx->d_fft;
     const FusedSchedule s = launch_schedule(sched, tiles, launch_tile);
-    CMD_HIP(launch_zero_u32(d_counter, 1, st));
-    CMD_HIP(launch_fused(a, s, ctx->n_cu, st));
+    {
+        TraceRange r("fused synd-bm-root");
+        CMD_HIP(launch_zero_u32(d_counter, 1, st));
+        CMD_HIP(launch_fused(a, s, ctx->n_cu, st));
+    }
     if (mode == CMD_MODE_POSITIONS) {
+        TraceRange r("positions");
         PositionsArgs pa;
         pa.count = (int32_t)n_items;
         pa.pad0 = 0;
@@ -387,7 +401,11 @@ int enqueue_fused(cmd_ctx *ctx, int mode, const FusedSchedule sched[], const std
     d.ok = kem.ok;
     d.s_out = nullptr;
     d.fft = ctx->d_fft;
-    CMD_HIP(launch_decap_check(d, st));
+    {
+        TraceRange r("kem check");
+        CMD_HIP(launch_decap_check(d, st));
+    }
+    TraceRange r("kem hash");
     KemHashArgs h;
     h.count = (int32_t)n_items;
     h.pad0 = 0;
@@ -764,6 +782,7 @@ extern "C" int cmd_key_load(cmd_ctx *ctx, int32_t param_id, const uint8_t *sk, s
         return set_error(CMD_ERR_CRYPTO, "secret key of %zu bytes, %d expected", sk_len, p.sk_bytes);
 
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    TraceRange tr("key load");
     CMD_HIP(hipSetDevice(ctx->device));
     // the Goppa coefficients, control bits and s go to the device, which expands them
     const size_t key_bytes = (size_t)2 * p.t + (size_t)p.cond_bytes + (size_t)p.n / 8;
@@ -902,6 +921,7 @@ static int make_batch(cmd_ctx *ctx, int32_t mode, size_t count, const int32_t *k
 static int run_batch(cmd_batch *b, const void *d_ct, void *d_e, hipStream_t st)
 {
     cmd_ctx *ctx = b->ctx;
+    TraceRange tr(b->use_graph ? "batch graph" : "batch");
     if (ctx->timing) CMD_HIP(hipEventRecord(ctx->ev0, st));
     if (b->use_graph) {
         if (!b->exec || b->graph_ct != d_ct || b->graph_e != d_e || b->graph_stream != st) {
@@ -1003,6 +1023,7 @@ extern "C" int cmd_decrypt(cmd_ctx *ctx, int32_t mode, size_t count, const int32
     if (!count) return CMD_OK;
     if (!ct || !e_out || !key_ids) return CMD_ERR_ARG;
     std::lock_guard<std::recursive_mutex> g(ctx->lock);
+    TraceRange tr("decrypt host buffers");
     CMD_HIP(hipSetDevice(ctx->device));
     if (mode == CMD_MODE_FUSED || mode == CMD_MODE_KEM || mode == CMD_MODE_POSITIONS)
         return io_decrypt(ctx, mode, count, key_ids, ct, ct_stride, e_out, e_stride);

@@ -77,6 +77,15 @@ __device__ __forceinline__ uint32_t wave_xor_all(uint32_t v)
                       __builtin_amdgcn_readlane((int)v, 32) ^ __builtin_amdgcn_readlane((int)v, 48));
 }
 
+// m ? a : b per bit, for a lane mask m (all-ones or zero per lane) held in a vector
+// register.  A select written on a lane condition can be compiled as a branch under an
+// execution mask, and a DPP move inside such a branch reads its disabled source lanes as
+// zero; this form always runs as one instruction on every lane.
+__device__ __forceinline__ uint32_t sel_mask(uint32_t m, uint32_t a, uint32_t b)
+{
+    return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
+}
+
 // XOR over the 64 lanes of each of N <= 16 values at once.  Each step folds one lane bit of
 // two registers into one: the permlane swaps exchange the halves (lane bit 5) or the odd and
 // even 16-lane rows (bit 4) of a register pair, so one XOR leaves the first value's partial
@@ -91,7 +100,7 @@ __device__ __forceinline__ constexpr int wave_xor_lane(int k)
 }
 
 template <int N>
-__device__ __forceinline__ void wave_xor_many(const uint32_t (&v)[N], uint32_t extra, bool lane_b2, bool lane_b3,
+__device__ __forceinline__ void wave_xor_many(const uint32_t (&v)[N], uint32_t extra, uint32_t m2, uint32_t m3,
                                               uint32_t (&out)[N])
 {
     static_assert(N >= 1 && N <= 16, "up to 16 values");
@@ -113,13 +122,13 @@ __device__ __forceinline__ void wave_xor_many(const uint32_t (&v)[N], uint32_t e
     for (int i = 0; i < N2; i++) {
         // lanes with bit 2 clear keep a and pair with a lane 4 away (bit 2 set) that sends its a
         const uint32_t a = r4[2 * i], b = 2 * i + 1 < N4 ? r4[2 * i + 1] : r4[2 * i];
-        const uint32_t keep = lane_b2 ? b : a, send = lane_b2 ? a : b;
+        const uint32_t keep = sel_mask(m2, b, a), send = sel_mask(m2, a, b);
         r2[i] = keep ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x124 /* row_ror:4 */, 0xF, 0xF, true);
     }
     uint32_t r;
     {
         const uint32_t a = r2[0], b = N2 > 1 ? r2[N2 - 1] : r2[0];
-        const uint32_t keep = lane_b3 ? b : a, send = lane_b3 ? a : b;
+        const uint32_t keep = sel_mask(m3, b, a), send = sel_mask(m3, a, b);
         r = keep ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x128 /* row_ror:8 */, 0xF, 0xF, true);
     }
     r ^= extra;
@@ -420,7 +429,7 @@ struct FftPlan {
     template <int LB, int Q>
     static __device__ __forceinline__ void swap_bits(uint32_t (&W)[WPL][M], int lane)
     {
-        [[maybe_unused]] const bool upper = (opaque(lane) >> LB) & 1;
+        [[maybe_unused]] const uint32_t upper = ((opaque(lane) >> LB) & 1) ? ~0u : 0u;
 #pragma unroll
         for (int r0 = 0; r0 < WPL; r0++) {
             if ((r0 >> Q) & 1) continue;
@@ -445,10 +454,10 @@ struct FftPlan {
                     W[r0][k] = (uint32_t)__builtin_amdgcn_update_dpp((int)a, (int)b, 0x124 /* row_ror:4 */, 0xF, 0xA, false);
                     W[r1][k] = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)a, 0x12C /* row_ror:12 */, 0xF, 0x5, false);
                 } else {
-                    const uint32_t send = upper ? a : b;
+                    const uint32_t send = sel_mask(upper, a, b);
                     const uint32_t recv = xor_lanes<(1 << LB)>(send);
-                    W[r0][k] = upper ? recv : a;
-                    W[r1][k] = upper ? b : recv;
+                    W[r0][k] = sel_mask(upper, recv, a);
+                    W[r1][k] = sel_mask(upper, b, recv);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -895,7 +904,7 @@ struct FusedTileKernel {
         uint32_t pm;         // planes of low(x) in lanes 0..M-1 (transposed top coefficient)
         uint32_t top_addr;   // ds_bpermute source of plane k of the top term: lane k ...
         uint32_t top_keep;   // ... for the lane wave_xor_lane(k) that reduces plane k
-        bool b2, b3;         // lane bits 2 and 3 (reduction selects)
+        uint32_t m2, m3;     // lane bits 2 and 3 as masks (reduction selects)
     };
 
     // Iteration n of the bit-sliced recurrence.  The wave-uniform per-ciphertext state (b, D
@@ -944,7 +953,7 @@ struct FusedTileKernel {
         }
         uint32_t part[M], d[M];
         bs_fold<M>(p, part);
-        wave_xor_many<M>(part, extra, ln.b2, ln.b3, d);
+        wave_xor_many<M>(part, extra, ln.m2, ln.m3, d);
         uint32_t mne = 0;
 #pragma unroll
         for (int k = 0; k < M; k++) mne |= d[k];
@@ -1040,8 +1049,8 @@ struct FusedTileKernel {
                 ln.top_addr = 4u * k;
                 ln.top_keep = ~0u;
             }
-        ln.b2 = (lane >> 2) & 1;
-        ln.b3 = (lane >> 3) & 1;
+        ln.m2 = ((lane >> 2) & 1) ? ~0u : 0u;
+        ln.m3 = ((lane >> 3) & 1) ? ~0u : 0u;
         uint32_t ct = 0, bt = 0;  // TOPX: planes of C_t and B_t, transposed
         const uint4 *rows4 = reinterpret_cast<const uint4 *>(lds);
 

@@ -68,60 +68,121 @@ static void ext_taps(int pidx, std::vector<ExtTap> &taps)
     }
 }
 
-// out = a * b mod F in GF(2^m)[y]/F(y), t coefficients each.
-static void ext_mul(const HostField &F, int t, const std::vector<ExtTap> &taps, const uint16_t *a,
-                    const uint16_t *b, uint16_t *out)
+// Carry-less products of one element with two packed in 32-bit halves (both fit: 2m-1 < 32).
+static inline uint64_t clmul_pair(uint32_t a, uint64_t b2)
 {
-    std::vector<uint16_t> prod(2 * t - 1, 0);
-    for (int i = 0; i < t; i++) {
-        if (!a[i]) continue;
-        for (int j = 0; j < t; j++) prod[i + j] ^= F.mul(a[i], b[j]);
+    return (uint64_t)_mm_cvtsi128_si64(_mm_clmulepi64_si128(_mm_cvtsi64_si128((long long)a),
+                                                            _mm_cvtsi64_si128((long long)b2), 0));
+}
+
+// Both halves of a clmul_pair result reduced modulo the field polynomial (two folds, as
+// HostField::reduce).
+static inline uint64_t reduce_pair(const HostField &F, uint64_t v)
+{
+    const uint64_t mask2 = (uint64_t)F.mask | ((uint64_t)F.mask << 32);
+    for (int r = 0; r < 2; r++) {
+        const uint64_t hi = (v >> F.m) & mask2;
+        v = (v & mask2) ^ clmul_pair(F.low, hi);
     }
+    return v;
+}
+
+// Folds the terms y^(2t-2) .. y^t of a reduced product with F(y) = y^t + taps.
+static void ext_fold(const HostField &F, int t, const std::vector<ExtTap> &taps, uint16_t *prod)
+{
     for (int i = 2 * t - 2; i >= t; i--) {
         uint16_t c = prod[i];
         if (!c) continue;
         for (const ExtTap &tp : taps) prod[i - t + tp.power] ^= F.mul(c, tp.coef);
     }
-    std::memcpy(out, prod.data(), sizeof(uint16_t) * t);
+}
+
+// out = a * b mod F in GF(2^m)[y]/F(y), t coefficients each: the coefficient products are
+// summed unreduced (two per carry-less multiply) and each sum reduced once.
+static void ext_mul(const HostField &F, int t, const std::vector<ExtTap> &taps, const uint16_t *a,
+                    const uint16_t *b, uint16_t *out)
+{
+    uint32_t acc[2 * kMaxT] = {};
+    uint64_t b2[kMaxT / 2 + 1];
+    for (int j = 0; j < t; j += 2) b2[j / 2] = b[j] | (j + 1 < t ? (uint64_t)b[j + 1] << 32 : 0ull);
+    for (int i = 0; i < t; i++) {
+        if (!a[i]) continue;
+        for (int j = 0; j < t; j += 2) {
+            const uint64_t pr = clmul_pair(a[i], b2[j / 2]);
+            acc[i + j] ^= (uint32_t)pr;
+            acc[i + j + 1] ^= (uint32_t)(pr >> 32);
+        }
+    }
+    uint16_t prod[2 * kMaxT];
+    for (int s = 0; s < 2 * t - 1; s++) prod[s] = F.reduce(acc[s]);
+    ext_fold(F, t, taps, prod);
+    std::memcpy(out, prod, sizeof(uint16_t) * t);
+}
+
+// out = a^2 mod F: squaring is linear in characteristic 2, (sum a_i y^i)^2 = sum a_i^2 y^2i.
+static void ext_sq(const HostField &F, int t, const std::vector<ExtTap> &taps, const uint16_t *a, uint16_t *out)
+{
+    uint16_t prod[2 * kMaxT] = {};
+    for (int i = 0; i < t; i++) prod[2 * i] = F.sq(a[i]);
+    ext_fold(F, t, taps, prod);
+    std::memcpy(out, prod, sizeof(uint16_t) * t);
 }
 
 // Minimal polynomial of f (t coefficients) over GF(2^m): g monic of degree t with
-// f^t + sum_{j<t} g_j f^j = 0.  Returns false when the powers are dependent.
+// f^t + sum_{j<t} g_j f^j = 0.  Returns false when the powers are dependent.  The columns
+// f^0 .. f^t (even powers by squaring an earlier column) form the system sum_j g_j f^j = f^t,
+// solved by forward elimination with the first nonzero pivot of each column and back
+// substitution; its solution is unique, so it is the same g as any other elimination order.
 static bool minimal_poly(int pidx, const uint16_t *f, uint16_t *g)
 {
-    const int t = kParams[pidx].t;
+    const int t = kParams[pidx].t, W = t + 1;
     HostField F(kParams[pidx].m);
     std::vector<ExtTap> taps;
     ext_taps(pidx, taps);
-    // columns: f^0 .. f^t, each t coefficients; matrix M[row r][col j] = (f^j)_r
-    std::vector<uint16_t> M((size_t)t * (t + 1), 0);
-    std::vector<uint16_t> cur(t, 0), nxt(t);
-    cur[0] = 1;
-    for (int j = 0; j <= t; j++) {
-        for (int r = 0; r < t; r++) M[(size_t)r * (t + 1) + j] = cur[r];
-        if (j < t) {
-            ext_mul(F, t, taps, cur.data(), f, nxt.data());
-            cur.swap(nxt);
-        }
+    // pw[j] = f^j (t coefficients each); M[row r][col j] = (f^j)_r
+    std::vector<uint16_t> pw((size_t)W * t, 0);
+    pw[0] = 1;
+    for (int j = 1; j <= t; j++) {
+        if (j % 2 == 0)
+            ext_sq(F, t, taps, &pw[(size_t)(j / 2) * t], &pw[(size_t)j * t]);
+        else
+            ext_mul(F, t, taps, &pw[(size_t)(j - 1) * t], f, &pw[(size_t)j * t]);
     }
-    // Solve sum_j g_j col_j = col_t by Gauss-Jordan (char 2: minus == plus).
+    std::vector<uint16_t> M((size_t)t * W);
+    for (int j = 0; j <= t; j++)
+        for (int r = 0; r < t; r++) M[(size_t)r * W + j] = pw[(size_t)j * t + r];
+    std::vector<uint64_t> prow2(W / 2 + 1);
     for (int c = 0; c < t; c++) {
         int piv = -1;
         for (int r = c; r < t; r++)
-            if (M[(size_t)r * (t + 1) + c]) { piv = r; break; }
+            if (M[(size_t)r * W + c]) { piv = r; break; }
         if (piv < 0) return false;
+        uint16_t *pr = &M[(size_t)c * W];
         if (piv != c)
-            for (int k = 0; k <= t; k++) std::swap(M[(size_t)piv * (t + 1) + k], M[(size_t)c * (t + 1) + k]);
-        uint16_t inv = F.inv(M[(size_t)c * (t + 1) + c]);
-        for (int k = 0; k <= t; k++) M[(size_t)c * (t + 1) + k] = F.mul(M[(size_t)c * (t + 1) + k], inv);
-        for (int r = 0; r < t; r++) {
-            if (r == c) continue;
-            uint16_t fac = M[(size_t)r * (t + 1) + c];
+            for (int k = c; k < W; k++) std::swap(M[(size_t)piv * W + k], pr[k]);
+        const uint16_t inv = F.inv(pr[c]);
+        for (int k = c; k < W; k++) pr[k] = F.mul(pr[k], inv);
+        // the pivot row from column c on, two entries per word
+        const int k0 = c, nk = W - c;
+        for (int q = 0; q < nk; q += 2)
+            prow2[q / 2] = pr[k0 + q] | (q + 1 < nk ? (uint64_t)pr[k0 + q + 1] << 32 : 0ull);
+        for (int r = c + 1; r < t; r++) {
+            uint16_t *row = &M[(size_t)r * W];
+            const uint16_t fac = row[c];
             if (!fac) continue;
-            for (int k = c; k <= t; k++) M[(size_t)r * (t + 1) + k] ^= F.mul(fac, M[(size_t)c * (t + 1) + k]);
+            for (int q = 0; q < nk; q += 2) {
+                const uint64_t v = reduce_pair(F, clmul_pair(fac, prow2[q / 2]));
+                row[k0 + q] ^= (uint16_t)v;
+                if (q + 1 < nk) row[k0 + q + 1] ^= (uint16_t)(v >> 32);
+            }
         }
     }
-    for (int r = 0; r < t; r++) g[r] = M[(size_t)r * (t + 1) + t];
+    // back substitution (unit diagonal): g_r = M[r][t] + sum_{k>r} M[r][k] g_k
+    for (int r = t - 1; r >= 0; r--) {
+        uint32_t acc = 0;
+        for (int k = r + 1; k < t; k++) acc ^= HostField::clmul(M[(size_t)r * W + k], g[k]);
+        g[r] = (uint16_t)(M[(size_t)r * W + t] ^ F.reduce(acc));
+    }
     return true;
 }
 
@@ -643,15 +704,27 @@ extern "C" int cmd_gen_keys_device(cmd_ctx *ctx, int32_t param_id, uint64_t base
             for (size_t q = 0; q < nb && !rc; q++) singular[idx[b0 + q]] = flags[q];
         }
         if (rc) break;
-        // per key, the first winning attempt in seed order
-        std::vector<size_t> winner(n_keys, SIZE_MAX);
+        // condition bits of every candidate still in the running (host threads), then per key
+        // the first winning attempt in seed order
         std::vector<std::vector<uint8_t>> conds(cands.size());
-        for (size_t i = 0; i < cands.size(); i++) {
-            const size_t k = cands[i].key;
-            if (winner[k] != SIZE_MAX || !kc[i].poly_ok || singular[i]) continue;
-            conds[i].assign(p.cond_bytes, 0);
-            if (controlbits(p.m, kc[i].pi, conds[i].data(), conds[i].size())) winner[k] = i;
+        std::vector<char> routed(cands.size(), 0);
+        {
+            std::atomic<size_t> next{0};
+            auto work = [&]() {
+                for (size_t i; (i = next.fetch_add(1)) < cands.size();) {
+                    if (!kc[i].poly_ok || singular[i]) continue;
+                    conds[i].assign(p.cond_bytes, 0);
+                    routed[i] = controlbits(p.m, kc[i].pi, conds[i].data(), conds[i].size());
+                }
+            };
+            std::vector<std::thread> pool;
+            for (int i = 1; i < threads && (size_t)i < cands.size(); i++) pool.emplace_back(work);
+            work();
+            for (auto &th : pool) th.join();
         }
+        std::vector<size_t> winner(n_keys, SIZE_MAX);
+        for (size_t i = 0; i < cands.size(); i++)
+            if (routed[i] && winner[cands[i].key] == SIZE_MAX) winner[cands[i].key] = i;
         for (size_t k = 0; k < n_keys; k++) {
             if (done[k]) continue;
             if (winner[k] != SIZE_MAX) {

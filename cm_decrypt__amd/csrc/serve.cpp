@@ -36,6 +36,7 @@
 #include <sys/prctl.h>
 
 #include "engine.h"
+#include "trace.h"
 
 using namespace cmd;
 
@@ -164,6 +165,7 @@ size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 // the end of s.reqs with CMD_ERR_KEY and left out of the launch.
 int launch_slot(cmd_server *srv, Slot &s)
 {
+    TraceRange tr("server launch");
     cmd_ctx *ctx = srv->ctx;
     std::vector<uint32_t> dec, kem, bad;
     dec.reserve(s.reqs.size());
