@@ -587,6 +587,9 @@ def main():
     ap.add_argument("--key-independence", action="store_true",
                     help="four fused launches of the same ciphertexts under two key assignments "
                          "(profile them with rocprofv3 --pmc) and exit")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="measure the VALU issue rate against wavefronts per SIMD and independent chains, "
+                         "and the dependent-step latencies, then exit")
     ap.add_argument("--stage-split", action="store_true",
                     help="time the headline launch with the stage-truncated libraries and exit")
     ap.add_argument("--fused-only", action="store_true",
@@ -610,6 +613,17 @@ def main():
 
     import torch
     import cm_decrypt__amd as cm
+
+    if args.issue_probe:
+        torch.cuda.init()
+        with cm.Context(0) as pctx:
+            rates = {f"{w}x{c}": round(pctx.valu_issue_rate(w, c), 4) for w in (1, 2, 3, 4, 8) for c in (1, 2, 8)}
+            lat = pctx.chain_latency()
+        print(json.dumps({"valu_issue_rate": {"unit": "wave instructions per SIMD per clock at the peak clock",
+                                              "key": "waves_per_simd x chains", "rates": rates},
+                          "chain_latency_ticks": {"xor": round(lat[0], 2), "dpp_xor": round(lat[1], 2),
+                                                  "readlane_xor": round(lat[2], 2)}}), flush=True)
+        return
 
     threads = max(1, min(args.threads, os.cpu_count() or 1))
     t_setup = time.perf_counter()

@@ -183,6 +183,8 @@ _sig("cmd_server_destroy", _I, _P)
 _sig("cmd_set_kernel_timing", _I, _P, _I32)
 _sig("cmd_kernel_timing", _I, _P, _c.POINTER(_c.c_double), _c.POINTER(_U64))
 _sig("cmd_valu_peak", _I, _P, _c.POINTER(_c.c_double))
+_sig("cmd_valu_issue_rate", _I, _P, _I32, _I32, _c.POINTER(_c.c_double))
+_sig("cmd_chain_latency", _I, _P, _c.POINTER(_c.c_double))
 _sig("cmd_build_info", _c.c_char_p)
 
 lib = _lib
@@ -551,6 +553,20 @@ class Context:
         v = _c.c_double(0)
         _check(_lib.cmd_valu_peak(self._h, _c.byref(v)), "cmd_valu_peak")
         return float(v.value)
+
+    def valu_issue_rate(self, waves_per_simd: int, chains: int) -> float:
+        """VALU wave instructions per SIMD per clock with waves_per_simd wavefronts per SIMD
+        running `chains` independent v_xor_b32 chains per lane."""
+        v = _c.c_double(0)
+        _check(_lib.cmd_valu_issue_rate(self._h, waves_per_simd, chains, _c.byref(v)), "cmd_valu_issue_rate")
+        return float(v.value)
+
+    def chain_latency(self):
+        """Dependent-step latencies of one wavefront in shader-clock ticks: (v_xor_b32, DPP row
+        shift + v_xor_b32, v_readlane + scalar + v_xor_b32)."""
+        t = (_c.c_double * 3)()
+        _check(_lib.cmd_chain_latency(self._h, t), "cmd_chain_latency")
+        return tuple(float(x) for x in t)
 
     def kernel_timing(self):
         us, n = _c.c_double(0), _U64(0)

@@ -366,6 +366,15 @@ int cmd_kernel_timing(cmd_ctx *ctx, double *avg_us, uint64_t *launches);
  * work is compared with.  No counterpart in the reference; a measurement aid. */
 int cmd_valu_peak(cmd_ctx *ctx, double *lane_ops_per_s);
 
+/* VALU issue rate in wave instructions per SIMD per clock (at the device's peak clock) with
+ * waves_per_simd (1..8) wavefronts per SIMD, each lane running `chains` (1, 2, 4 or 8)
+ * independent chains of v_xor_b32; and the dependent-chain latencies of one wavefront in
+ * shader-clock ticks per step (v_xor_b32, DPP row shift + v_xor_b32, v_readlane + scalar +
+ * v_xor_b32).  No counterpart in the reference; measurement aids for what bounds a
+ * wavefront's issue. */
+int cmd_valu_issue_rate(cmd_ctx *ctx, int32_t waves_per_simd, int32_t chains, double *wave_insts_per_clk);
+int cmd_chain_latency(cmd_ctx *ctx, double ticks[3]);
+
 /* Library build identification (kernel variants compiled in). */
 const char *cmd_build_info(void);
 
