@@ -512,6 +512,61 @@ struct FftPlan {
         }
     }
 
+    // Multiplier index of the first word pair (register 0) of level Dl.
+    template <int Dl>
+    static __device__ __forceinline__ uint32_t first_g_index(int lane)
+    {
+        constexpr int J = M - 6 - Dl;
+        constexpr Layout LAY = SCHED.at[J - JMIN];
+        return lane_part(LAY, opaque(lane), J);
+    }
+
+    // butterfly<Dl> with the multiplier loads one word pair ahead: G holds the multipliers of
+    // this level's first pair on entry (loaded by the caller) and those of the next level's
+    // first pair on return, so each product runs while the next pair's planes are in flight
+    // instead of waiting for its own.
+    template <int Dl>
+    static __device__ __forceinline__ void butterfly_ahead(uint32_t (&W)[WPL][M], int lane, const FftTab *tab,
+                                                           uint32_t (&G)[M])
+    {
+        constexpr int J = M - 6 - Dl;
+        constexpr int LI = J - JMIN;
+        constexpr int SL = SCHED.swap_lane[LI], SQ = SCHED.swap_q[LI];
+        if constexpr (SL >= 0) swap_bits<(SL >= 0 ? SL : 0), (SQ >= 0 ? SQ : 0)>(W, lane);
+        constexpr int RB = 1 << SCHED.bq[LI];
+        constexpr Layout LAY = SCHED.at[LI];
+        asm volatile("" ::: "memory");
+        const uint32_t wl = lane_part(LAY, opaque(lane), J);
+#pragma unroll
+        for (int r = 0; r < WPL; r++) {
+            if (r & RB) continue;
+            int rn = -1;  // the next pair of this level
+#pragma unroll
+            for (int q = WPL - 1; q > r; q--)
+                if (!(q & RB)) rn = q;
+            uint32_t Gn[M];
+            const bool ahead = rn >= 0 || Dl > 0;
+            if (rn >= 0)
+                load_g<Dl>(tab, wl | reg_part(LAY, rn, J), Gn);
+            else if constexpr (Dl > 0)
+                load_g<Dl - 1>(tab, first_g_index<Dl - 1>(lane), Gn);
+            uint32_t t[M];
+            bs_mul<M>(G, W[r | RB], t);
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                W[r][k] ^= t[k];
+                W[r | RB][k] ^= W[r][k];
+            }
+            if (ahead) {
+#pragma unroll
+                for (int k = 0; k < M; k++) G[k] = Gn[k];
+            }
+            // one word pair at a time: interleaving the products would multiply the live
+            // registers
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
     // Full word index of lane l's register r in the final layout.
     static __device__ __forceinline__ uint32_t final_word(uint32_t wl_fin, int r)
     {
@@ -1333,6 +1388,9 @@ struct FusedTileKernel {
             const int item = a.items[tl.first + x];
             const KeyDev *kd = &a.keys[a.item_key[tl.first + x]];
             uint8_t *eo = a.e + (size_t)item * a.e_stride;
+            // the first level's multipliers load while the leaves are read
+            uint32_t Gm[M];
+            FP::template load_g<LOG - 1>(tab, FP::template first_g_index<LOG - 1>(lane), Gm);
             uint32_t W[WPL][M];
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
@@ -1341,14 +1399,14 @@ struct FusedTileKernel {
 #pragma unroll
                 for (int k = 0; k < M; k++) W[r][k] = bitmask(v, k);
             }
-            if constexpr (LOG == 8) FP::template butterfly<7>(W, lane, tab);
-            FP::template butterfly<6>(W, lane, tab);
-            FP::template butterfly<5>(W, lane, tab);
-            FP::template butterfly<4>(W, lane, tab);
-            FP::template butterfly<3>(W, lane, tab);
-            FP::template butterfly<2>(W, lane, tab);
-            FP::template butterfly<1>(W, lane, tab);
-            FP::template butterfly<0>(W, lane, tab);
+            if constexpr (LOG == 8) FP::template butterfly_ahead<7>(W, lane, tab, Gm);
+            FP::template butterfly_ahead<6>(W, lane, tab, Gm);
+            FP::template butterfly_ahead<5>(W, lane, tab, Gm);
+            FP::template butterfly_ahead<4>(W, lane, tab, Gm);
+            FP::template butterfly_ahead<3>(W, lane, tab, Gm);
+            FP::template butterfly_ahead<2>(W, lane, tab, Gm);
+            FP::template butterfly_ahead<1>(W, lane, tab, Gm);
+            FP::template butterfly_ahead<0>(W, lane, tab, Gm);
             uint32_t z[WPL];
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
