@@ -574,14 +574,17 @@ struct FftPlan {
     }
 
     template <int K>
-    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], gu32 *masks, int lane,
-                                                       uint32_t wl_fin)
+    static __device__ __forceinline__ void benes_masks(gu32 *masks, uint32_t wl_fin, uint32_t (&mw)[WPL])
     {
-        constexpr int S = K < M ? K : 2 * M - 2 - K;
         gu32 *mrow = masks + K * NW + opaque((int)wl_fin);
-        uint32_t mw[WPL];
 #pragma unroll
         for (int r = 0; r < WPL; r++) mw[r] = mrow[reg_part(SCHED.fin, r, 64)];
+    }
+
+    template <int K>
+    static __device__ __forceinline__ void benes_apply(uint32_t (&z)[WPL], const uint32_t (&mw)[WPL])
+    {
+        constexpr int S = K < M ? K : 2 * M - 2 - K;
         if constexpr (S < 5) {
             constexpr int SH = 1 << S;
 #pragma unroll
@@ -614,11 +617,28 @@ struct FftPlan {
         }
     }
 
-    template <int... Ks>
-    static __device__ __forceinline__ void benes_all(uint32_t (&z)[WPL], gu32 *masks, int lane,
-                                                     uint32_t wl_fin, std::integer_sequence<int, Ks...>)
+    template <int K>
+    static __device__ __forceinline__ void benes_layer(uint32_t (&z)[WPL], gu32 *masks, int, uint32_t wl_fin)
     {
-        (benes_layer<Ks>(z, masks, lane, wl_fin), ...);
+        uint32_t mw[WPL];
+        benes_masks<K>(masks, wl_fin, mw);
+        benes_apply<K>(z, mw);
+    }
+
+    // Layers K .. 2M-2 in order, given layer K's masks; each layer's masks load while the
+    // layer before it is applied.
+    template <int K>
+    static __device__ __forceinline__ void benes_from(uint32_t (&z)[WPL], gu32 *masks, uint32_t wl_fin,
+                                                      const uint32_t (&mw)[WPL])
+    {
+        if constexpr (K + 1 <= 2 * M - 2) {
+            uint32_t mn[WPL];
+            benes_masks<K + 1>(masks, wl_fin, mn);
+            benes_apply<K>(z, mw);
+            benes_from<K + 1>(z, masks, wl_fin, mn);
+        } else {
+            benes_apply<K>(z, mw);
+        }
     }
 
     // Transpose of butterfly<Dl>: (a, b) -> (a + b, G (a + b) + b), then the lane/register
@@ -1406,6 +1426,10 @@ struct FusedTileKernel {
             FP::template butterfly_ahead<3>(W, lane, tab, Gm);
             FP::template butterfly_ahead<2>(W, lane, tab, Gm);
             FP::template butterfly_ahead<1>(W, lane, tab, Gm);
+            // the Benes network's first masks load during the last butterfly level
+            gu32 *bmask = gptr(kd->benes);
+            uint32_t mw0[WPL];
+            FP::template benes_masks<0>(bmask, wl_fin, mw0);
             FP::template butterfly_ahead<0>(W, lane, tab, Gm);
             uint32_t z[WPL];
 #pragma unroll
@@ -1415,7 +1439,7 @@ struct FusedTileKernel {
                 for (int k = 0; k < M; k++) nz |= W[r][k];
                 z[r] = ~nz;
             }
-            FP::benes_all(z, gptr(kd->benes), lane, wl_fin, std::make_integer_sequence<int, 2 * M - 1>{});
+            FP::template benes_from<0>(z, bmask, wl_fin, mw0);
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
                 const int g = (int)FP::final_word(wl_fin, r);
