@@ -271,7 +271,17 @@ struct FftPlan {
     template <int D, int LANEX, int PBX>
     static __device__ __forceinline__ void radix_pass(uint32_t (&F)[NP][M], int lane, int s, int sel)
     {
-        // XOR into targets (bits s-1, s-2 of the node-local index == sel) the values at p ^ (LANEX + 64 PBX)
+        // XOR into targets (bits s-1, s-2 of the node-local index == sel) the values at p ^ (LANEX + 64 PBX).
+        // No source is a target of the same pass, so every exchange is issued before the first
+        // XOR: the lane moves through LDS (lane distances 16 and 32) are then in flight together
+        // instead of one round trip each.
+        uint32_t X[NP][M];
+#pragma unroll
+        for (int pb = 0; pb < NP; pb++) {
+            if ((pb ^ PBX) >= NP) continue;
+#pragma unroll
+            for (int k = 0; k < M; k++) X[pb][k] = xor_lanes_any<LANEX>(F[pb ^ PBX][k]);
+        }
 #pragma unroll
         for (int pb = 0; pb < NP; pb++) {
             const int p = lane + 64 * pb;
@@ -279,7 +289,7 @@ struct FftPlan {
             const uint32_t hit = (((i >> (s - 2)) & 3) == sel) ? ~0u : 0u;
             if ((pb ^ PBX) >= NP) continue;
 #pragma unroll
-            for (int k = 0; k < M; k++) F[pb][k] ^= xor_lanes_any<LANEX>(F[pb ^ PBX][k]) & hit;
+            for (int k = 0; k < M; k++) F[pb][k] ^= X[pb][k] & hit;
         }
     }
 
