@@ -148,10 +148,42 @@ def generator_leg(cm, ctx, kids, ct, e_true, key_of, seed, cpu_s, threads):
             "gpu_s": round(gpu_s, 3), "identical": bool(same)}
 
 
-def host_io_leg(cm, ctx, key_ids, ct, e_true, reps, steps):
+def pcie_copy_rates(torch, ct_bytes, e_bytes, steps):
+    """Raw pinned host<->device copy rates for the byte counts one host-buffer call moves:
+    ciphertexts host->device, error bitmaps device->host, each alone and both at once on two
+    streams (the overlap the chunked pipeline relies on). GB/s = 1e9 bytes/s."""
+    h_in = torch.empty(ct_bytes, dtype=torch.uint8, pin_memory=True)
+    h_out = torch.empty(e_bytes, dtype=torch.uint8, pin_memory=True)
+    d_in = torch.empty(ct_bytes, dtype=torch.uint8, device="cuda")
+    d_out = torch.empty(e_bytes, dtype=torch.uint8, device="cuda")
+    s_in, s_out = torch.cuda.Stream(), torch.cuda.Stream()
+
+    def timed(h2d, d2h):
+        def once():
+            if h2d:
+                with torch.cuda.stream(s_in):
+                    d_in.copy_(h_in, non_blocking=True)
+            if d2h:
+                with torch.cuda.stream(s_out):
+                    h_out.copy_(d_out, non_blocking=True)
+            torch.cuda.synchronize()
+        once()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            once()
+        return (time.perf_counter() - t0) / steps
+
+    w_in, w_out, w_both = timed(True, False), timed(False, True), timed(True, True)
+    return {"h2d_gbs": round(ct_bytes / w_in / 1e9, 2), "d2h_gbs": round(e_bytes / w_out / 1e9, 2),
+            "both_ms": round(w_both * 1e3, 3), "h2d_ms": round(w_in * 1e3, 3),
+            "d2h_ms": round(w_out * 1e3, 3), "bytes_h2d": int(ct_bytes), "bytes_d2h": int(e_bytes)}
+
+
+def host_io_leg(cm, torch, ctx, key_ids, ct, e_true, reps, steps):
     """PCIe-inclusive decryption from host buffers (cmd_decrypt): the headline batch repeated
     `reps` times in one call, so the chunked pipeline overlaps copies and kernels; once with
-    pageable numpy buffers (staged) and once with cmd_host_alloc buffers (direct DMA)."""
+    pageable numpy buffers (staged) and once with cmd_host_alloc buffers (direct DMA). Beside
+    them, the raw pinned copy time of the same bytes, the floor any host-buffer call has."""
     n = len(ct) * reps
     kids = np.tile(key_ids, reps)
     big = np.tile(ct, (reps, 1))
@@ -190,6 +222,9 @@ def host_io_leg(cm, ctx, key_ids, ct, e_true, reps, steps):
                                "bit_exact": bool(exact)}
     for a in (h_ct, h_e, h_p):
         ctx.host_free(a)
+    copy = pcie_copy_rates(torch, big.nbytes, want.nbytes, steps)
+    copy["decrypt_ceiling"] = round(n / (copy["both_ms"] / 1e3), 1)
+    out["pcie_copy"] = copy
     return out
 
 
@@ -696,11 +731,14 @@ def main():
 
     host_io = None
     if not args.no_host_io:
-        host_io = host_io_leg(cm, ctx, key_ids, ct, e_true, 4, max(2, args.steps // 3))
+        host_io = host_io_leg(cm, torch, ctx, key_ids, ct, e_true, 4, max(2, args.steps // 3))
         log(f"host buffers, {host_io['items']} items per call: pageable {host_io['pageable']['value']:,.0f} dec/s, "
             f"pinned {host_io['pinned']['value']:,.0f} dec/s, pinned positions "
             f"{host_io['pinned_positions']['value']:,.0f} dec/s exact="
-            f"{all(host_io[k]['bit_exact'] for k in ('pageable', 'pinned', 'pinned_positions'))}")
+            f"{all(host_io[k]['bit_exact'] for k in ('pageable', 'pinned', 'pinned_positions'))}; raw pinned "
+            f"copies h2d {host_io['pcie_copy']['h2d_gbs']} GB/s, d2h {host_io['pcie_copy']['d2h_gbs']} GB/s, "
+            f"both at once {host_io['pcie_copy']['both_ms']} ms per call "
+            f"(ceiling {host_io['pcie_copy']['decrypt_ceiling']:,.0f} dec/s)")
 
     plain = None
     if not args.no_plain:

@@ -107,12 +107,12 @@ static int upload_key_slot(cmd_ctx *ctx, int32_t id)
 }
 
 // Waits for the work of the context's own streams (its default stream and the host-buffer
-// pipeline's slots); other streams of the device are left alone.
+// pipeline's copy and kernel streams); other streams of the device are left alone.
 static void ctx_quiesce(cmd_ctx *ctx)
 {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (IoSlot &s : ctx->io)
-        if (s.st) (void)hipStreamSynchronize(s.st);
+    for (hipStream_t st : {ctx->io_copy, ctx->io_kern})
+        if (st) (void)hipStreamSynchronize(st);
 }
 
 // ---- batch planning ---------------------------------------------------------------------
@@ -137,40 +137,49 @@ int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, co
                std::vector<FusedTile> &tiles, std::vector<int32_t> &items, std::vector<int32_t> &item_key,
                int *launch_tile)
 {
+    // per key: its set, or -1 when the slot holds no key
+    std::vector<int8_t> kp(ctx->keys.size(), -1);
+    for (size_t k = 0; k < kp.size(); k++)
+        if (ctx->keys[k].used) kp[k] = (int8_t)ctx->keys[k].pidx;
+    // items ordered by set, heavier sets (larger index here) first, then by key id, then by
+    // position: a counting sort over key ids, linear in the items
+    std::vector<size_t> at(ctx->keys.size(), 0);
     for (size_t i = 0; i < count; i++) {
-        const int32_t k = key_ids[i];
-        if (k < 0 || (size_t)k >= ctx->keys.size() || !ctx->keys[k].used || ctx->keys[k].pidx < 0)
-            return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)k);
+        const uint32_t k = (uint32_t)key_ids[i];
+        if (k >= kp.size() || kp[k] < 0)
+            return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)key_ids[i]);
+        at[k]++;
     }
-    std::vector<int32_t> order(count);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
-        int pa = ctx->keys[key_ids[a]].pidx, pc = ctx->keys[key_ids[c]].pidx;
-        if (pa != pc) return pa > pc;  // heavier sets (larger index here) first
-        return key_ids[a] < key_ids[c];
-    });
+    size_t pos = 0, set_end[kNumParams] = {};
+    for (int pidx = kNumParams - 1; pidx >= 0; pidx--) {
+        for (size_t k = 0; k < at.size(); k++)
+            if (kp[k] == pidx) {
+                const size_t c = at[k];
+                at[k] = pos;
+                pos += c;
+            }
+        set_end[pidx] = pos;
+    }
+    items.resize(count);
+    item_key.resize(count);
+    for (size_t i = 0; i < count; i++) items[at[(size_t)key_ids[i]]++] = (int32_t)i;
+    for (size_t i = 0; i < count; i++) item_key[i] = key_ids[items[i]];
     tiles.clear();
-    items.clear();
-    item_key.clear();
     int widest = 0;
     // tiles of up to tile_cts consecutive items of one set (a tile may span keys of that set)
-    for (size_t i = 0; i < count;) {
-        const int pidx = ctx->keys[key_ids[order[i]]].pidx;
-        const int width = sched[pidx].tile_cts;
-        size_t j = i;
-        while (j < count && (int)(j - i) < width && ctx->keys[key_ids[order[j]]].pidx == pidx) j++;
-        FusedTile t;
-        t.pidx = pidx;
-        t.first = (int32_t)items.size();
-        t.count = (int32_t)(j - i);
-        t.pad = 0;
-        tiles.push_back(t);
-        widest = std::max(widest, width);
-        for (size_t q = i; q < j; q++) {
-            items.push_back(order[q]);
-            item_key.push_back(key_ids[order[q]]);
+    size_t set_begin = 0;
+    for (int pidx = kNumParams - 1; pidx >= 0; pidx--) {
+        const size_t width = (size_t)sched[pidx].tile_cts;
+        if (set_end[pidx] > set_begin) widest = std::max(widest, sched[pidx].tile_cts);
+        for (size_t i = set_begin; i < set_end[pidx]; i += width) {
+            FusedTile t;
+            t.pidx = pidx;
+            t.first = (int32_t)i;
+            t.count = (int32_t)std::min(width, set_end[pidx] - i);
+            t.pad = 0;
+            tiles.push_back(t);
         }
-        i = j;
+        set_begin = set_end[pidx];
     }
     if (launch_tile) *launch_tile = widest ? widest : 32;
     return CMD_OK;
@@ -186,11 +195,15 @@ FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<Fus
 void fit_tiles(const cmd_ctx *ctx, const FusedSchedule in[], size_t count, const int32_t *key_ids,
                FusedSchedule out[])
 {
-    size_t per_set[kNumParams] = {};
+    // items per set, counted per key first
+    std::vector<size_t> per_key(ctx->keys.size(), 0);
     for (size_t i = 0; i < count; i++) {
-        const int32_t k = key_ids[i];
-        if (k >= 0 && (size_t)k < ctx->keys.size() && ctx->keys[k].used) per_set[ctx->keys[k].pidx]++;
+        const uint32_t k = (uint32_t)key_ids[i];
+        if (k < per_key.size()) per_key[k]++;
     }
+    size_t per_set[kNumParams] = {};
+    for (size_t k = 0; k < per_key.size(); k++)
+        if (per_key[k] && ctx->keys[k].used && ctx->keys[k].pidx >= 0) per_set[ctx->keys[k].pidx] += per_key[k];
     for (int i = 0; i < kNumParams; i++) out[i] = in[i];
     const size_t target = (size_t)ctx->n_cu * 4;
     for (;;) {

@@ -49,14 +49,23 @@ struct Engine;
 // chunk and pinned staging.
 constexpr size_t kIoSlots = 3;
 struct IoSlot {
-    hipStream_t st = nullptr;
-    hipEvent_t done = nullptr, t0 = nullptr, t1 = nullptr;  // t0/t1 bracket the kernels when timing
+    // in_done: copy in finished; k_done: kernels finished; done: copy out finished;
+    // t0/t1 bracket the kernels when timing
+    hipEvent_t in_done = nullptr, k_done = nullptr, done = nullptr, t0 = nullptr, t1 = nullptr;
     uint8_t *h_ct = nullptr, *h_out = nullptr;
     char *h_plan = nullptr;
     size_t h_ct_cap = 0, h_out_cap = 0, h_plan_cap = 0;
     DevBuf d_ct, d_out, d_plan, d_kem_e, d_ok, d_msg;
     bool busy = false, staged_out = false, timed = false;
-    size_t first = 0, count = 0;
+    bool out_pending = false;   // kernels enqueued, copy out not yet enqueued
+    bool mark_pending = false;  // copy out enqueued, `done` not yet recorded
+    uint8_t *dst = nullptr;    // where the copy out goes (caller's buffer or h_out)
+    size_t first = 0, count = 0, out_bytes = 0;
+    // the chunk's plan: schedules, tiles, and the offsets of its parts in the plan image
+    FusedSchedule sched[kNumParams];
+    std::vector<FusedTile> tiles;
+    int launch_tile = 32;
+    size_t off_keys = 0, off_tiles = 0, off_ctr = 0;
 };
 
 }  // namespace cmd
@@ -81,6 +90,7 @@ struct cmd_ctx {
     cmd::FftTab *d_fft = nullptr;   // kFftVariants additive-FFT constant tables
 
     std::vector<cmd::IoSlot> io;                            // host-buffer pipeline stages
+    hipStream_t io_copy = nullptr, io_kern = nullptr;       // the pipeline's copies; its kernels
     std::vector<std::pair<uintptr_t, size_t>> host_pinned;  // cmd_host_alloc blocks
 
     // host plan of cmd_decrypt_device_noalloc, reused from call to call

@@ -11,9 +11,18 @@ namespace cmd {
 class TraceRange {
 public:
     explicit TraceRange(const char *name) { roctxRangePushA(name); }
-    ~TraceRange() { roctxRangePop(); }
+    ~TraceRange() { end(); }
+    // closes the range before the end of the scope (once)
+    void end()
+    {
+        if (open_) roctxRangePop();
+        open_ = false;
+    }
     TraceRange(const TraceRange &) = delete;
     TraceRange &operator=(const TraceRange &) = delete;
+
+private:
+    bool open_ = true;
 };
 
 }  // namespace cmd

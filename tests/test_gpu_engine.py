@@ -275,6 +275,20 @@ def test_kernel_timing(ctx, keys):
     assert n >= 1 and us > 0
 
 
+def test_issue_probes_report_plausible_rates(ctx):
+    """The VALU issue-rate and dependent-latency probes: rates are fractions of one instruction
+    per clock, a second wavefront per SIMD raises the rate, and a readlane round trip through
+    the scalar unit is slower than a DPP step, which is slower than a plain VALU step."""
+    one, two = ctx.valu_issue_rate(1, 8), ctx.valu_issue_rate(2, 8)
+    assert 0.02 < one < 1.5 and 0.02 < two < 1.5
+    assert two > 1.3 * one
+    xor, dpp, rdl = ctx.chain_latency()
+    assert 0 < xor < dpp < rdl
+    for bad in ((0, 8), (9, 8), (1, 3)):
+        with pytest.raises(cm.CmdError):
+            ctx.valu_issue_rate(*bad)
+
+
 def _lbits_from_L(m, L, stride):
     out = np.zeros((m, stride), np.uint32)
     for k in range(m):
@@ -377,6 +391,29 @@ def test_host_pipeline_chunks_pageable_and_pinned(ctx, keys):
         for i in (0, 1, 4098, 4099, 65535, 65536, 69999):
             assert k[i].tobytes() == hashlib.shake_256(b"\x01" + want[i, :p.e_bytes].tobytes()
                                                        + ct[i].tobytes()).digest(32)
+    finally:
+        ctx.host_free(h_ct)
+        ctx.host_free(h_e)
+
+
+def test_host_pipeline_mixed_sets_across_chunks(ctx, keys):
+    """A shuffled five-set batch of three chunks through the host-buffer pipeline: each chunk
+    has its own plan (sets, tiles, schedules) in its slot while the next is planned, and every
+    row equals its encrypted error, pageable and pinned."""
+    ct, kids, want = mixed_rows(keys, 2000, 4242)
+    reps = 14  # 140000 items -> 3 chunks
+    ct, kids, want = np.tile(ct, (reps, 1)), np.tile(kids, reps), np.tile(want, (reps, 1))
+    order = np.random.default_rng(7).permutation(len(ct))
+    ct, kids, want = ct[order], kids[order], want[order]
+    print(f"mixed host pipeline items={len(ct)}", flush=True)
+    assert np.array_equal(ctx.decrypt(kids, ct), want)
+    h_ct = ctx.host_array(ct.shape)
+    h_e = ctx.host_array(want.shape)
+    try:
+        h_ct[:] = ct
+        h_e[:] = 0x5A
+        ctx.decrypt(kids, h_ct, out=h_e)
+        assert np.array_equal(h_e, want)
     finally:
         ctx.host_free(h_ct)
         ctx.host_free(h_e)
