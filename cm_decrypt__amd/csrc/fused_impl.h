@@ -1874,10 +1874,25 @@ constexpr int kPairLds = kWaveLds + kPairXchWords + 16;
 static_assert(kWaveLds % 4 == 0, "exchange slots 16-byte aligned");
 
 
-// Persistent kernel: each wavefront pulls tiles until the list is exhausted.  Dynamic LDS of
-// kWaveLds words per wavefront; the workgroup size (64..256) gives the wavefronts per group.
-// OCC is the number of wavefronts per SIMD the build is register-limited for: 1 (up to 512
-// VGPR+AGPR per lane) or 2 (256).
+// Workgroups are dispatched to the XCDs in turn (workgroup b to XCD b % kXcds), and each XCD
+// has its own L2.  The first tile of every wavefront is assigned statically so that the
+// wavefronts of one XCD take one contiguous range of the tile list, which the host sorts by
+// set and key: each XCD then reads the tables of few keys into its L2 instead of all of them.
+// Wavefront (b, wv) takes index (prefix(b % kXcds) + b / kXcds) * waves + wv, prefix(x) being
+// the number of groups on XCDs below x: a bijection of the grid's wavefronts onto [0, W).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_first_tile(uint32_t b, uint32_t groups, uint32_t waves, uint32_t wv)
+{
+    const uint32_t x = b % kXcds, per = groups / kXcds, extra = groups % kXcds;
+    const uint32_t prefix = x * per + (x < extra ? x : extra);
+    return (prefix + b / kXcds) * waves + wv;
+}
+
+// Persistent kernel: each wavefront takes its first tile by the XCD-major assignment above,
+// then pulls tiles W, W+1, ... from the counter (W = wavefronts in the grid) until the list is
+// exhausted.  Dynamic LDS of kWaveLds words per wavefront; the workgroup size (64..256) gives
+// the wavefronts per group.  OCC is the number of wavefronts per SIMD the build is
+// register-limited for: 1 (up to 512 VGPR+AGPR per lane) or 2 (256).
 template <int ALG, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC))) void k_fused(FusedArgs a)
 {
@@ -1885,10 +1900,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t *lds = lds_dyn + wv * kWaveLds;
     const bool e_aligned = ((a.e_stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(a.e) & 3u) == 0);
-    for (;;) {
-        uint32_t tid = 0;
-        if (lane == 0) tid = atomicAdd(a.tile_counter, 1u);
-        tid = (uint32_t)__builtin_amdgcn_readfirstlane((int)tid);
+    const uint32_t waves = blockDim.x >> 6, n_waves = gridDim.x * waves;
+    uint32_t tid = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (int)xcd_first_tile(blockIdx.x, gridDim.x, waves, (uint32_t)wv));
+    for (bool first = true;; first = false) {
+        if (!first) {
+            uint32_t q = 0;
+            if (lane == 0) q = atomicAdd(a.tile_counter, 1u);
+            tid = n_waves + (uint32_t)__builtin_amdgcn_readfirstlane((int)q);
+        }
         if (tid >= (uint32_t)a.n_tiles) break;
         const FusedTile tl = a.tiles[tid];
         switch (tl.pidx) {
