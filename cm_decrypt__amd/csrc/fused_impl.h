@@ -1002,10 +1002,14 @@ struct FusedTileKernel {
                                                    uint32_t &ct, uint32_t &bt, const BmLanes &ln)
     {
         constexpr bool FULL = LO >= NFULL;
-        // discrepancy d = sum_i C_i s_{n-i} (rows hold s_{j} at j+1, row 0 is zero)
-        uint32_t p[2 * M - 1];
+        constexpr int KL = Kara<M>::L, NQ0 = Kara<M>::NQ0, NQ2 = Kara<M>::NQ2;
+        // discrepancy d = sum_i C_i s_{n-i} (rows hold s_{j} at j+1, row 0 is zero), its
+        // products accumulated as Karatsuba half products and recombined once
+        uint32_t q0[NQ0], q1[NQ0], q2[NQ2];
 #pragma unroll
-        for (int s2 = 0; s2 < 2 * M - 1; s2++) p[s2] = 0;
+        for (int s2 = 0; s2 < NQ0; s2++) q0[s2] = q1[s2] = 0;
+#pragma unroll
+        for (int s2 = 0; s2 < NQ2; s2++) q2[s2] = 0;
 #pragma unroll
         for (int b = 0; b < NBV; b++) {
             if (!mac_on<LO, HI>(b, n)) continue;
@@ -1017,8 +1021,13 @@ struct FusedTileKernel {
                 const uint4 w = rows4[row * 4 + q];
                 sv[4 * q] = w.x; sv[4 * q + 1] = w.y; sv[4 * q + 2] = w.z; sv[4 * q + 3] = w.w;
             }
-            bs_mac_wide<M>(C[b], sv, p);
+            uint32_t cs[KL], ss[KL];
+            kara_half_sum<M>(C[b], cs);
+            kara_half_sum<M>(sv, ss);
+            kara_acc<M>(C[b], sv, cs, ss, q0, q1, q2);
         }
+        uint32_t p[2 * M - 1];
+        kara_combine<M>(q0, q1, q2, p);
         // + C_t s_{n-t} (TOPX): s_{n-t} (row n-t+1; row 0, zero, while n < t) as wave-uniform
         // planes, Horner over them; plane k of the result is in lane k, and goes to the lane
         // that reduces plane k of the discrepancy
@@ -1052,11 +1061,26 @@ struct FusedTileKernel {
             for (int a = M - 2; a >= 0; a--) acc = and_xor(and_xor(tx_mulx_v(acc, ln.pm), ct, bb[a]), bt, d[a]);
             ct = acc;
         }
+        // b*C + d*B per block as Karatsuba half products (the half sums of b and d are
+        // wave-uniform, made once), one recombination and one fold per block
+        uint32_t bbs[KL], ds[KL];
+        kara_half_sum<M>(bb, bbs);
+        kara_half_sum<M>(d, ds);
 #pragma unroll
         for (int b = NBV - 1; b >= 0; b--) {
             if (!mac_on<LO, HI>(b, n)) continue;
-            uint32_t cn[M];
-            bs_mul_add2<M>(bb, C[b], d, B[b], cn);
+            uint32_t u0[NQ0], u1[NQ0], u2[NQ2], cs[KL], bs[KL];
+#pragma unroll
+            for (int s2 = 0; s2 < NQ0; s2++) u0[s2] = u1[s2] = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < NQ2; s2++) u2[s2] = 0;
+            kara_half_sum<M>(C[b], cs);
+            kara_half_sum<M>(B[b], bs);
+            kara_acc<M>(bb, C[b], bbs, cs, u0, u1, u2);
+            kara_acc<M>(d, B[b], ds, bs, u0, u1, u2);
+            uint32_t cw[2 * M - 1], cn[M];
+            kara_combine<M>(u0, u1, u2, cw);
+            bs_fold<M>(cw, cn);
 #pragma unroll
             for (int k = 0; k < M; k++) {
                 B[b][k] = __builtin_amdgcn_bitop3_b32(mle, C[b][k], B[b][k], 0xCA);  // mle ? C : B
