@@ -635,19 +635,23 @@ struct FftPlan {
         benes_apply<K>(z, mw);
     }
 
-    // Layers K .. 2M-2 in order, given layer K's masks; each layer's masks load while the
-    // layer before it is applied.
+    // Layers K .. 2M-2 in order with three layers' masks in flight: on entry m0, m1, m2 hold
+    // the masks of layers K, K+1, K+2 (those that exist); layer K+3's load is issued before
+    // layer K is applied, so each load has three layers' work to land in.
     template <int K>
-    static __device__ __forceinline__ void benes_from(uint32_t (&z)[WPL], gu32 *masks, uint32_t wl_fin,
-                                                      const uint32_t (&mw)[WPL])
+    static __device__ __forceinline__ void benes_from3(uint32_t (&z)[WPL], gu32 *masks, uint32_t wl_fin,
+                                                       const uint32_t (&m0)[WPL], const uint32_t (&m1)[WPL],
+                                                       const uint32_t (&m2)[WPL])
     {
-        if constexpr (K + 1 <= 2 * M - 2) {
+        constexpr int LAST = 2 * M - 2;
+        if constexpr (K + 3 <= LAST) {
             uint32_t mn[WPL];
-            benes_masks<K + 1>(masks, wl_fin, mn);
-            benes_apply<K>(z, mw);
-            benes_from<K + 1>(z, masks, wl_fin, mn);
+            benes_masks<K + 3>(masks, wl_fin, mn);
+            benes_apply<K>(z, m0);
+            benes_from3<K + 1>(z, masks, wl_fin, m1, m2, mn);
         } else {
-            benes_apply<K>(z, mw);
+            benes_apply<K>(z, m0);
+            if constexpr (K + 1 <= LAST) benes_from3<K + 1>(z, masks, wl_fin, m1, m2, m2);
         }
     }
 
@@ -1465,6 +1469,9 @@ struct FusedTileKernel {
             uint32_t mw0[WPL];
             FP::template benes_masks<0>(bmask, wl_fin, mw0);
             FP::template butterfly_ahead<0>(W, lane, tab, Gm);
+            uint32_t mw1[WPL], mw2[WPL];
+            FP::template benes_masks<1>(bmask, wl_fin, mw1);
+            FP::template benes_masks<2>(bmask, wl_fin, mw2);
             uint32_t z[WPL];
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
@@ -1473,7 +1480,7 @@ struct FusedTileKernel {
                 for (int k = 0; k < M; k++) nz |= W[r][k];
                 z[r] = ~nz;
             }
-            FP::template benes_from<0>(z, bmask, wl_fin, mw0);
+            FP::template benes_from3<0>(z, bmask, wl_fin, mw0, mw1, mw2);
 #pragma unroll
             for (int r = 0; r < WPL; r++) {
                 const int g = (int)FP::final_word(wl_fin, r);
