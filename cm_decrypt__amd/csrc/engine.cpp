@@ -137,32 +137,45 @@ int plan_fused(const cmd_ctx *ctx, const FusedSchedule sched[], size_t count, co
                std::vector<FusedTile> &tiles, std::vector<int32_t> &items, std::vector<int32_t> &item_key,
                int *launch_tile)
 {
-    // per key: its set, or -1 when the slot holds no key
-    std::vector<int8_t> kp(ctx->keys.size(), -1);
-    for (size_t k = 0; k < kp.size(); k++)
-        if (ctx->keys[k].used) kp[k] = (int8_t)ctx->keys[k].pidx;
-    // items ordered by set, heavier sets (larger index here) first, then by key id, then by
-    // position: a counting sort over key ids, linear in the items
-    std::vector<size_t> at(ctx->keys.size(), 0);
+    const size_t K = ctx->keys.size();
     for (size_t i = 0; i < count; i++) {
         const uint32_t k = (uint32_t)key_ids[i];
-        if (k >= kp.size() || kp[k] < 0)
+        if (k >= K || !ctx->keys[k].used || ctx->keys[k].pidx < 0)
             return set_error(CMD_ERR_KEY, "item %zu: key id %d is not loaded", i, (int)key_ids[i]);
-        at[k]++;
     }
-    size_t pos = 0, set_end[kNumParams] = {};
-    for (int pidx = kNumParams - 1; pidx >= 0; pidx--) {
-        for (size_t k = 0; k < at.size(); k++)
-            if (kp[k] == pidx) {
-                const size_t c = at[k];
-                at[k] = pos;
-                pos += c;
-            }
-        set_end[pidx] = pos;
-    }
+    // items ordered by set, heavier sets (larger index here) first, then by key id, then by
+    // position; set_end[p] is where set p's items end in that order
     items.resize(count);
     item_key.resize(count);
-    for (size_t i = 0; i < count; i++) items[at[(size_t)key_ids[i]]++] = (int32_t)i;
+    size_t set_end[kNumParams] = {};
+    if (count < K) {
+        // few items against many key slots (serving launches): a stable comparison sort
+        size_t per_set[kNumParams] = {};
+        for (size_t i = 0; i < count; i++) per_set[ctx->keys[key_ids[i]].pidx]++;
+        size_t pos = 0;
+        for (int pidx = kNumParams - 1; pidx >= 0; pidx--) set_end[pidx] = pos += per_set[pidx];
+        std::iota(items.begin(), items.end(), 0);
+        std::stable_sort(items.begin(), items.end(), [&](int32_t a, int32_t c) {
+            const int pa = ctx->keys[key_ids[a]].pidx, pc = ctx->keys[key_ids[c]].pidx;
+            if (pa != pc) return pa > pc;
+            return key_ids[a] < key_ids[c];
+        });
+    } else {
+        // a counting sort over key ids, linear in the items and the key slots
+        std::vector<size_t> at(K, 0);
+        for (size_t i = 0; i < count; i++) at[(size_t)key_ids[i]]++;
+        size_t pos = 0;
+        for (int pidx = kNumParams - 1; pidx >= 0; pidx--) {
+            for (size_t k = 0; k < K; k++)
+                if (ctx->keys[k].used && ctx->keys[k].pidx == pidx) {
+                    const size_t c = at[k];
+                    at[k] = pos;
+                    pos += c;
+                }
+            set_end[pidx] = pos;
+        }
+        for (size_t i = 0; i < count; i++) items[at[(size_t)key_ids[i]]++] = (int32_t)i;
+    }
     for (size_t i = 0; i < count; i++) item_key[i] = key_ids[items[i]];
     tiles.clear();
     int widest = 0;
@@ -195,15 +208,22 @@ FusedSchedule launch_schedule(const FusedSchedule sched[], const std::vector<Fus
 void fit_tiles(const cmd_ctx *ctx, const FusedSchedule in[], size_t count, const int32_t *key_ids,
                FusedSchedule out[])
 {
-    // items per set, counted per key first
-    std::vector<size_t> per_key(ctx->keys.size(), 0);
-    for (size_t i = 0; i < count; i++) {
-        const uint32_t k = (uint32_t)key_ids[i];
-        if (k < per_key.size()) per_key[k]++;
-    }
     size_t per_set[kNumParams] = {};
-    for (size_t k = 0; k < per_key.size(); k++)
-        if (per_key[k] && ctx->keys[k].used && ctx->keys[k].pidx >= 0) per_set[ctx->keys[k].pidx] += per_key[k];
+    if (count < ctx->keys.size()) {
+        for (size_t i = 0; i < count; i++) {
+            const uint32_t k = (uint32_t)key_ids[i];
+            if (k < ctx->keys.size() && ctx->keys[k].used && ctx->keys[k].pidx >= 0) per_set[ctx->keys[k].pidx]++;
+        }
+    } else {
+        // many items: counted per key first, then per set
+        std::vector<size_t> per_key(ctx->keys.size(), 0);
+        for (size_t i = 0; i < count; i++) {
+            const uint32_t k = (uint32_t)key_ids[i];
+            if (k < per_key.size()) per_key[k]++;
+        }
+        for (size_t k = 0; k < per_key.size(); k++)
+            if (per_key[k] && ctx->keys[k].used && ctx->keys[k].pidx >= 0) per_set[ctx->keys[k].pidx] += per_key[k];
+    }
     for (int i = 0; i < kNumParams; i++) out[i] = in[i];
     const size_t target = (size_t)ctx->n_cu * 4;
     for (;;) {

@@ -45,8 +45,9 @@ struct KeySlot {
 
 struct Engine;
 
-// One stage of the host-buffer pipeline (hostio.cpp): a stream, the device buffers of one
-// chunk and pinned staging.
+// One stage of the host-buffer pipeline (hostio.cpp): the device buffers of one chunk, pinned
+// staging, the chunk's plan and the events that order it between the pipeline's copy and
+// kernel streams.
 constexpr size_t kIoSlots = 3;
 struct IoSlot {
     // in_done: copy in finished; k_done: kernels finished; done: copy out finished;

@@ -74,7 +74,6 @@ int host_reserve(void *&ptr, size_t &cap, size_t bytes)
     return CMD_OK;
 }
 
-// Copies a finished slot's results to the caller (staged chunks only).
 // Enqueues a slot's copy out on the copy stream, behind its kernels; io_mark_done records its
 // completion event.  The two are apart so that the record can go behind the next chunk's
 // kernels (see above).
@@ -107,6 +106,8 @@ int io_mark_done(cmd_ctx *ctx, IoSlot &s)
     return CMD_OK;
 }
 
+// Waits for a slot's chunk (enqueuing its copy out first if still pending) and copies staged
+// results to the caller.
 int io_finish(cmd_ctx *ctx, IoSlot &s, uint8_t *out, size_t out_stride)
 {
     if (!s.busy) return CMD_OK;
