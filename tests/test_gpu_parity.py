@@ -189,6 +189,26 @@ def test_weight_below_t(ctx, keys, pid):
         assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("wpg,gpc", [(1, 1), (2, 1), (1, 3)])
+def test_more_tiles_than_wavefronts(ctx, keys, wpg, gpc):
+    """Launches with more tiles than the grid has wavefronts: each wavefront's first tile comes
+    from the XCD-major static assignment, the rest from the counter past the grid; every tile
+    is decrypted once (grids of n_CU x gpc workgroups of one or two wavefronts, so 256 to 768
+    wavefronts for 625 tiles)."""
+    pid = 348864
+    sk, kid = keys[pid]
+    ct, e = cm.gen_encrypt(pid, sk, 4321, 20000)  # 625 tiles of 32
+    saved = ctx.schedule(pid)
+    try:
+        ctx.set_tile_fit(False)
+        ctx.set_schedule(pid, tile_cts=32, waves_per_group=wpg, waves_per_ct=1, groups_per_cu=gpc, root_alg=1)
+        got = ctx.decrypt([kid] * len(ct), ct, mode=cm.MODE_FUSED)
+    finally:
+        ctx.set_tile_fit(True)
+        ctx.set_schedule(pid, **saved)
+    assert np.array_equal(got, e)
+
+
 @pytest.mark.parametrize("alg", [0, 1])
 @pytest.mark.parametrize("tile,wpg,gpc", [(16, 1, 2), (32, 2, 2), (32, 4, 1), (32, 1, 8), (16, 2, 4), (32, 4, 2),
                                           (4, 2, 4), (1, 4, 1)])
