@@ -1239,10 +1239,10 @@ struct FusedTileKernel {
         for (int k = 0; k < M; k++) colb[k] = mle ? cold[k] : colb[k];
     }
 
-    static __device__ __forceinline__ void berlekamp_massey_scalar(uint32_t *lds, int lane, int count,
-                                                                   uint32_t (&C)[NBL][M])
+    // Scalar syndromes of the first SCALAR_BM slots out of the planes, as uint16 rows
+    // (2t per slot) at the start of the wave's LDS.
+    static __device__ __forceinline__ void scalar_syndrome_rows(uint32_t *lds, int lane)
     {
-        // scalar syndromes of the first SCALAR_BM slots out of the planes, then as uint16 rows
         uint32_t sx[SCALAR_BM][NQ];
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -1268,6 +1268,13 @@ struct FusedTileKernel {
                 if (j < 2 * T) S[x * 2 * T + j] = (uint16_t)sx[x][q];
             }
         wave_sync();
+    }
+
+    static __device__ __forceinline__ void berlekamp_massey_scalar(uint32_t *lds, int lane, int count,
+                                                                   uint32_t (&C)[NBL][M])
+    {
+        scalar_syndrome_rows(lds, lane);
+        const uint16_t *S = reinterpret_cast<const uint16_t *>(lds);
 
 #pragma unroll
         for (int b = 0; b < NBL; b++)
@@ -1293,6 +1300,120 @@ struct FusedTileKernel {
             for (int b = 0; b < NBL; b++)
 #pragma unroll
                 for (int k = 0; k < M; k++) C[b][k] |= ((c[b] >> k) & 1u) << x;
+        }
+    }
+
+    // ---- the per-ciphertext recurrence on the two wavefronts of a pair -----------------------
+    // Wavefront 0 holds lane block 0 of c and B, wavefront 1 the blocks above it.  Every
+    // iteration each wavefront sums its blocks' products and posts the wave-uniform partial sum
+    // in an exchange slot, wavefront 0 also its pre-update c and B of lane 63 (the selected B
+    // of that lane is what the shift moves into block 1); after one barrier both form the same
+    // discrepancy and keep the same wave-uniform state (L, b, its multiplication table).  Slots
+    // alternate with the iteration's parity, so a slot is rewritten only after the other
+    // wavefront has passed the barrier that follows its read.
+    static constexpr int SX_SLOT = 8;  // words per slot: partial sum of wave 0 | of wave 1 | c_63 | B_63
+    template <int W>
+    static __device__ __forceinline__ void scalar_bm_step_pair(int n, const uint16_t *Sx, uint32_t *xch, int lane,
+                                                               uint32_t keep_last, uint32_t (&c)[NBL],
+                                                               uint32_t (&bv)[NBL], uint32_t &bb, uint32_t &L,
+                                                               uint32_t (&colb)[M])
+    {
+        uint32_t part = 0;
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+            if ((W == 0) != (b == 0) || !mac_on<0, 2 * T>(b, n)) continue;
+            const int j = n - (lane + 64 * b);
+            const uint32_t sv = j >= 0 ? (uint32_t)Sx[j] : 0u;
+            part ^= gf_clmul<M>(c[b], sv);
+        }
+        const uint32_t pw = uni(wave_xor_all(part));
+        uint32_t *slot = xch + (n & 1) * SX_SLOT;
+        if (lane == 0) slot[W] = pw;
+        if constexpr (W == 0) {
+            const uint32_t c63 = (uint32_t)__builtin_amdgcn_readlane((int)c[0], 63);
+            const uint32_t b63 = (uint32_t)__builtin_amdgcn_readlane((int)bv[0], 63);
+            if (lane == 0) {
+                slot[2] = c63;
+                slot[3] = b63;
+            }
+        }
+        __syncthreads();
+        const uint32_t d = gf_reduce<M>(pw ^ uni(slot[1 - W]));
+        [[maybe_unused]] uint32_t c63 = 0, b63 = 0;
+        if constexpr (W == 1) {
+            c63 = uni(slot[2]);
+            b63 = uni(slot[3]);
+        }
+        const bool mle = d != 0 && 2 * L <= (uint32_t)n;
+        uint32_t cold[M];
+        gf_mul_cols<M>(d, cold);
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+            if ((W == 0) != (b == 0) || !mac_on<0, 2 * T>(b, n)) continue;
+            const uint32_t cn = gf_mul_by_cols<M>(colb, c[b]) ^ gf_mul_by_cols<M>(cold, bv[b]);
+            bv[b] = mle ? c[b] : bv[b];
+            c[b] = cn;
+        }
+#pragma unroll
+        for (int b = NBL - 1; b >= 0; b--) {
+            if ((W == 0) != (b == 0) || !shift_on<0, 2 * T>(b, n)) continue;
+            uint32_t carry = 0;
+            if (b == 1 && W == 1)
+                carry = mle ? c63 : b63;  // wavefront 0's lane 63 after the selection
+            else if (b > 1)
+                carry = (uint32_t)__builtin_amdgcn_readlane((int)bv[b > 0 ? b - 1 : 0], 63);
+            bv[b] = shift_up1(bv[b], carry);
+        }
+        if constexpr (W == 1) bv[NBL - 1] &= keep_last;
+        // wave-uniform selects, no branch on the secret mle
+        L = mle ? (uint32_t)n + 1 - L : L;
+        bb = mle ? d : bb;
+#pragma unroll
+        for (int k = 0; k < M; k++) colb[k] = mle ? cold[k] : colb[k];
+    }
+
+    // berlekamp_massey_scalar on the two wavefronts of a pair: wavefront 0 extracts the scalar
+    // syndromes, both run the recurrence, and each writes its own blocks' coefficient planes.
+    template <int W>
+    static __device__ __forceinline__ void berlekamp_massey_scalar_pair(uint32_t *lds, uint32_t *xch, int lane,
+                                                                        int count)
+    {
+        static_assert(NBL >= 2, "wavefront 1 holds the blocks above block 0");
+        if constexpr (W == 0) scalar_syndrome_rows(lds, lane);
+        __syncthreads();
+        const uint16_t *S = reinterpret_cast<const uint16_t *>(lds);
+        uint32_t C[NBL][M];
+#pragma unroll
+        for (int b = 0; b < NBL; b++)
+#pragma unroll
+            for (int k = 0; k < M; k++) C[b][k] = 0;
+        const uint32_t keep_last = (lane + 64 * (NBL - 1) <= T) ? ~0u : 0u;
+        for (int x = 0; x < count; x++) {
+            const uint16_t *Sx = S + x * 2 * T;
+            uint32_t c[NBL], bv[NBL];
+#pragma unroll
+            for (int b = 0; b < NBL; b++) {
+                c[b] = (W == 0 && b == 0 && lane == 0) ? 1u : 0u;   // C = 1
+                bv[b] = (W == 0 && b == 0 && lane == 1) ? 1u : 0u;  // B = x
+            }
+            uint32_t bb = 1, L = 0;
+            uint32_t colb[M];
+            gf_mul_cols<M>(bb, colb);
+#pragma unroll 1
+            for (int n = 0; n < 2 * T; n++) scalar_bm_step_pair<W>(n, Sx, xch, lane, keep_last, c, bv, bb, L, colb);
+#pragma unroll
+            for (int b = 0; b < NBL; b++)
+#pragma unroll
+                for (int k = 0; k < M; k++) C[b][k] |= ((c[b] >> k) & 1u) << x;
+        }
+        __syncthreads();  // both wavefronts past their last syndrome read: the rows may go
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+            if ((W == 0) != (b == 0)) continue;
+            const int i = lane + 64 * b;
+            if (i > T) continue;
+#pragma unroll
+            for (int k = 0; k < M; k++) lds[i * kRowWords + k] = C[b][k];
         }
     }
 
@@ -1879,12 +2000,10 @@ struct FusedTileKernel {
         static_assert(ALG == 1, "two-wavefront tiles use the additive-FFT root search");
         syndrome_pair(a, tl, lds, lane, wv);
         if (tl.count <= SCALAR_BM) {
-            if (wv == 0) {
-                uint32_t C[NBL][M];
-                berlekamp_massey_scalar(lds, lane, tl.count, C);
-                wave_sync();
-                write_coef_planes(C, lds, lane);
-            }
+            if (wv == 0)
+                berlekamp_massey_scalar_pair<0>(lds, xch, lane, tl.count);
+            else
+                berlekamp_massey_scalar_pair<1>(lds, xch, lane, tl.count);
         } else if (wv == 0) {
             berlekamp_massey_pair<0>(lds, xch, lane);
         } else {

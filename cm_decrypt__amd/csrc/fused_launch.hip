@@ -41,13 +41,25 @@ hipError_t launch_fused(const FusedArgs &a, const FusedSchedule &s, int n_cu, hi
     if (!fused_supported(s)) return hipErrorInvalidValue;
     if (a.n_tiles <= 0) return hipSuccess;
     if (s.waves_per_ct == 2) return launch_fused_fft_pair(a, s, n_cu, st);
+    const bool small = (long)a.n_tiles < (long)n_cu * s.waves_per_group * s.groups_per_cu;
+    // Small launches of tiles of at most two ciphertexts (serving at low load) decode each
+    // ciphertext with the per-ciphertext recurrence, a chain of dependent steps that one
+    // wavefront issues slowly: the paired kernel splits it, and the syndrome, over the two
+    // wavefronts of a workgroup.
+    if (small && s.tile_cts <= 2 && s.root_alg == 1) {
+        FusedSchedule p = s;
+        p.waves_per_ct = 2;
+        p.waves_per_group = 2;
+        p.groups_per_cu = s.waves_per_group * s.groups_per_cu / 2 > 0 ? s.waves_per_group * s.groups_per_cu / 2 : 1;
+        return launch_fused_fft_pair(a, p, n_cu, st);
+    }
     // more than one wavefront per SIMD needs the 256-register build
     const bool occ2 = s.waves_per_group * s.groups_per_cu > 4;
     // Fewer tiles than resident wavefronts (small and serving launches): one wavefront per
     // workgroup, so the tiles land on as many CUs as there are tiles instead of sharing a few
     // CUs' scalar units, LDS and caches; the build and the wavefronts per CU stay the same.
     FusedSchedule t = s;
-    if ((long)a.n_tiles < (long)n_cu * s.waves_per_group * s.groups_per_cu) {
+    if (small) {
         t.groups_per_cu = s.waves_per_group * s.groups_per_cu;
         t.waves_per_group = 1;
     }
